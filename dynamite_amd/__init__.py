@@ -1,0 +1,295 @@
+"""dynamite_amd - MI355X (gfx950) per-RK-step renormalized DMFT update.
+
+Thin ctypes binding of dynamite_amd/libdmfe.so (C ABI in include/dmfe.h) used by
+the tests, smoke() and bench.py.  There is no CPU fallback: every compute call
+goes to the HIP library, and importing a missing library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdmfe.so")
+
+DMFE_OK = 0
+INTERP_POLY, INTERP_RATIONAL, INTERP_BSPLINE = 0, 1, 2
+RK54, SSPRK104 = 1, 2
+POS_BSEARCH, POS_ISEARCH, POS_SAME = 0, 1, 2
+
+FIELD_NAMES = ("QKA1", "QRA1", "QKA2", "QRA2", "QKB1", "QRB1", "QKB2", "QRB2",
+               "SigmaKA1", "SigmaRA1", "SigmaRA2", "SigmaKB1", "SigmaRB1", "SigmaKB2", "SigmaRB2")
+
+_lib = None
+
+
+class DmfeError(RuntimeError):
+    pass
+
+
+class GridSpec(C.Structure):
+    _fields_ = [("N", C.c_int64), ("Tmax", C.c_double), ("alpha", C.c_double), ("delta", C.c_double),
+                ("spline_order", C.c_int32), ("method", C.c_int32), ("order", C.c_int32),
+                ("fh_stencil", C.c_int32)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("N", C.c_int64), ("m", C.c_int64), ("theta", C.c_void_p), ("integ", C.c_void_p),
+                ("phi2", C.c_void_p), ("inds", C.c_void_p * 3), ("weights", C.c_void_p * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [("p", C.c_int32), ("p2", C.c_int32), ("lambda_", C.c_double), ("T0", C.c_double),
+                ("Gamma", C.c_double), ("delta_t_min", C.c_double), ("delta_max", C.c_double),
+                ("rmax", C.c_double * 11), ("log_response", C.c_int32), ("reserved", C.c_int32)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def lib():
+    """Loads libdmfe.so (build it with `python -m dynamite_amd.build`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DmfeError(f"{LIB_PATH} is missing: run `python -m dynamite_amd.build` (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        L.dmfe_last_error.restype = C.c_char_p
+        L.dmfe_version.restype = C.c_char_p
+        L.dmfe_grid_stencil_width.restype = C.c_int64
+        L.dmfe_grid_stencil_width.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != DMFE_OK:
+        raise DmfeError(f"{what} failed ({rc}): {lib().dmfe_last_error().decode()}")
+
+
+class GridPackage:
+    """In-memory Grid_data/<L> package (theta, int.dat, phi1/phi2, pos*, inds*, weights*)."""
+
+    def __init__(self, N, m, theta, integ, phi2, inds, weights, phi1=None, pos=None, spec=None):
+        self.N, self.m = int(N), int(m)
+        self.theta = np.ascontiguousarray(theta, np.float64)
+        self.integ = np.ascontiguousarray(integ, np.float64)
+        self.phi2 = np.ascontiguousarray(phi2, np.float64)
+        self.inds = np.ascontiguousarray(inds, np.int64).reshape(3, self.N * self.N)
+        self.weights = np.ascontiguousarray(weights, np.float64).reshape(3, self.N * self.N * self.m)
+        self.phi1 = phi1
+        self.pos = pos
+        self.spec = spec
+
+    def as_struct(self):
+        g = Grid()
+        g.N, g.m = self.N, self.m
+        g.theta, g.integ, g.phi2 = _ptr(self.theta), _ptr(self.integ), _ptr(self.phi2)
+        for f in range(3):
+            g.inds[f] = self.inds[f].ctypes.data
+            g.weights[f] = self.weights[f].ctypes.data
+        return g
+
+    def params_text(self):
+        s = self.spec
+        if s is None:
+            return None
+        meth = ("poly", "rational", "bspline")[s.method]
+        txt = ("# DYNAMITE grid generation parameters\n"
+               f"len={s.N}\nTmax={s.Tmax:.17e}\nspline_order={s.spline_order}\n"
+               f"interp_method={meth}\ninterp_order={s.order}\n")
+        if s.method == INTERP_RATIONAL:
+            txt += f"fh_stencil={self.m}\n"
+        txt += f"alpha={s.alpha:.17e}\ndelta={s.delta:.17e}\nsubdir={s.N}\n"
+        return txt
+
+    def write(self, directory):
+        os.makedirs(directory, exist_ok=True)
+        pos = self.pos if self.pos is not None else (None, None, None)
+        txt = self.params_text()
+        rc = lib().dmfe_grid_write_dir(directory.encode(), C.c_int64(self.N), C.c_int64(self.m), _ptr(self.theta),
+                                       _ptr(self.integ), _ptr(self.phi1), _ptr(self.phi2), _ptr(pos[0]), _ptr(pos[1]),
+                                       _ptr(pos[2]), _ptr(self.inds), _ptr(self.weights),
+                                       txt.encode() if txt else None)
+        _check(rc, "dmfe_grid_write_dir")
+
+
+def stencil_width(N, method=INTERP_POLY, order=9, fh_stencil=0):
+    return int(lib().dmfe_grid_stencil_width(N, method, order, fh_stencil))
+
+
+def generate_grid(N, Tmax=1e5, spline_order=5, method=INTERP_POLY, order=9, fh_stencil=0, alpha=0.0, delta=0.0,
+                  with_pos=False):
+    """Grid package generator (the reference's `grid` subcommand, grid_cli.cpp:35-233)."""
+    spec = GridSpec(N, Tmax, alpha, delta, spline_order, method, order, fh_stencil)
+    m = stencil_width(N, method, order, fh_stencil)
+    P = N * N
+    theta, integ = np.zeros(N), np.zeros(N)
+    phi1, phi2 = np.zeros(P), np.zeros(P)
+    pos = (np.zeros(P), np.zeros(P), np.zeros(P)) if with_pos else (None, None, None)
+    inds = np.zeros(3 * P, np.int64)
+    weights = np.zeros(3 * P * m)
+    rc = lib().dmfe_grid_generate(C.byref(spec), _ptr(theta), _ptr(integ), _ptr(phi1), _ptr(phi2), _ptr(pos[0]),
+                                  _ptr(pos[1]), _ptr(pos[2]), _ptr(inds), _ptr(weights))
+    _check(rc, "dmfe_grid_generate")
+    return GridPackage(N, m, theta, integ, phi2, inds, weights, phi1=phi1, pos=pos if with_pos else None, spec=spec)
+
+
+def read_grid(directory):
+    """Loads a Grid_data/<L> directory (import(), io_input.cpp:90-110)."""
+    N, m = C.c_int64(), C.c_int64()
+    _check(lib().dmfe_grid_read_info(directory.encode(), C.byref(N), C.byref(m)), "dmfe_grid_read_info")
+    N, m = N.value, m.value
+    P = N * N
+    theta, integ, phi2 = np.zeros(N), np.zeros(N), np.zeros(P)
+    inds, weights = np.zeros(3 * P, np.int64), np.zeros(3 * P * m)
+    rc = lib().dmfe_grid_read_dir(directory.encode(), C.c_int64(N), C.c_int64(m), _ptr(theta), _ptr(integ),
+                                  _ptr(phi2), _ptr(inds), _ptr(weights))
+    _check(rc, "dmfe_grid_read_dir")
+    return GridPackage(N, m, theta, integ, phi2, inds, weights)
+
+
+def default_params(**kw):
+    p = Params()
+    lib().dmfe_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, "lambda_" if k == "lam" or k == "lambda_" else k, v)
+    return p
+
+
+class Context:
+    """One quench instance on the current HIP device (dmfe_ctx)."""
+
+    def __init__(self, grid, params, rows_hint=256):
+        self.grid = grid
+        self.N = grid.N
+        self._h = C.c_void_p()
+        self._gs = grid.as_struct()
+        _check(lib().dmfe_ctx_create(C.byref(self._gs), C.byref(params), C.c_int64(rows_hint), C.byref(self._h)),
+               "dmfe_ctx_create")
+
+    def close(self):
+        if self._h:
+            lib().dmfe_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state -------------------------------------------------------------------
+    def init_quench(self):
+        _check(lib().dmfe_init_quench(self._h), "dmfe_init_quench")
+
+    @property
+    def T1(self):
+        t = C.c_int64()
+        _check(lib().dmfe_get_T1(self._h, C.byref(t)), "dmfe_get_T1")
+        return t.value
+
+    def set_state(self, t1grid, QK, QR, dQK, dQR, rvec, drvec, dtratio=None):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (t1grid, QK, QR, dQK, dQR, rvec, drvec)]
+        dtr = None if dtratio is None else np.ascontiguousarray(dtratio, np.float64)
+        T1 = len(arrs[0])
+        _check(lib().dmfe_set_state(self._h, C.c_int64(T1), *[_ptr(a) for a in arrs], _ptr(dtr)), "dmfe_set_state")
+
+    def get_state(self):
+        T1, N = self.T1, self.N
+        out = {k: np.zeros(T1) for k in ("t1grid", "rvec", "drvec", "dtratio")}
+        for k in ("QK", "QR", "dQK", "dQR"):
+            out[k] = np.zeros(T1 * N)
+        _check(lib().dmfe_get_state(self._h, _ptr(out["t1grid"]), _ptr(out["QK"]), _ptr(out["QR"]), _ptr(out["dQK"]),
+                                    _ptr(out["dQR"]), _ptr(out["rvec"]), _ptr(out["drvec"]), _ptr(out["dtratio"])),
+               "dmfe_get_state")
+        return out
+
+    def set_control(self, delta_t, delta, delta_old, loop, method):
+        _check(lib().dmfe_set_control(self._h, C.c_double(delta_t), C.c_double(delta), C.c_double(delta_old),
+                                      C.c_int64(loop), C.c_int(method)), "dmfe_set_control")
+
+    def get_control(self):
+        dt, de, do, lp, me = C.c_double(), C.c_double(), C.c_double(), C.c_int64(), C.c_int()
+        _check(lib().dmfe_get_control(self._h, C.byref(dt), C.byref(de), C.byref(do), C.byref(lp), C.byref(me)),
+               "dmfe_get_control")
+        return dict(delta_t=dt.value, delta=de.value, delta_old=do.value, loop=lp.value, method=me.value)
+
+    # -- hot path ----------------------------------------------------------------
+    def interpolate(self, mode=POS_BSEARCH):
+        _check(lib().dmfe_interpolate(self._h, C.c_int(mode)), "dmfe_interpolate")
+
+    def positions(self):
+        b1, b2 = np.zeros(self.N), np.zeros(self.N * self.N)
+        _check(lib().dmfe_get_positions(self._h, _ptr(b1), _ptr(b2)), "dmfe_get_positions")
+        return b1, b2
+
+    def set_positions(self, b1, b2):
+        b1 = np.ascontiguousarray(b1, np.float64)
+        b2 = np.ascontiguousarray(b2, np.float64)
+        _check(lib().dmfe_set_positions(self._h, _ptr(b1), _ptr(b2)), "dmfe_set_positions")
+
+    def fields(self):
+        P = self.N * self.N
+        out = np.zeros(15 * P + self.N)
+        _check(lib().dmfe_get_fields(self._h, _ptr(out)), "dmfe_get_fields")
+        d = {name: out[i * P:(i + 1) * P] for i, name in enumerate(FIELD_NAMES)}
+        d["rInt"] = out[15 * P:]
+        return d
+
+    def rhs(self):
+        hK, hR = np.zeros(self.N), np.zeros(self.N)
+        _check(lib().dmfe_rhs(self._h, _ptr(hK), _ptr(hR)), "dmfe_rhs")
+        return hK, hR
+
+    def rstep(self):
+        r = C.c_double()
+        _check(lib().dmfe_rstep(self._h, C.byref(r)), "dmfe_rstep")
+        return r.value
+
+    def rk_step(self):
+        e = C.c_double()
+        _check(lib().dmfe_rk_step(self._h, C.byref(e)), "dmfe_rk_step")
+        return e.value
+
+    def loop_iteration(self):
+        t = C.c_double()
+        _check(lib().dmfe_loop_iteration(self._h, C.byref(t)), "dmfe_loop_iteration")
+        return t.value
+
+    def run(self, tmax, max_loop, corr_cap=0):
+        corr = np.zeros(3 * max(corr_cap, 1))
+        steps = C.c_int64()
+        _check(lib().dmfe_run(self._h, C.c_double(tmax), C.c_int64(max_loop), _ptr(corr) if corr_cap else None,
+                              C.c_int64(corr_cap), C.byref(steps)), "dmfe_run")
+        n = min(steps.value, corr_cap)
+        return steps.value, corr[:3 * n].reshape(n, 3)
+
+    def observables(self, t):
+        e, q = C.c_double(), C.c_double()
+        _check(lib().dmfe_observables(self._h, C.c_double(t), C.byref(e), C.byref(q)), "dmfe_observables")
+        return e.value, q.value
+
+    def sparsify(self, threshold):
+        _check(lib().dmfe_sparsify(self._h, C.c_double(threshold)), "dmfe_sparsify")
+
+    def synchronize(self):
+        _check(lib().dmfe_synchronize(self._h), "dmfe_synchronize")
+
+
+def profile(ctx, enable=True):
+    _check(lib().dmfe_profile(ctx._h, C.c_int(1 if enable else 0)), "dmfe_profile")
+
+
+def profile_read(ctx, kind):
+    """(total_ms, launches) of kernel `kind` (0 search, 1 rhs_rows, 2 finish) since profile(ctx)."""
+    ms, n = C.c_double(), C.c_int64()
+    _check(lib().dmfe_profile_read(ctx._h, C.c_int(kind), C.byref(ms), C.byref(n)), "dmfe_profile_read")
+    return ms.value, n.value

@@ -1,0 +1,807 @@
+// Host side of the step path: context (device tables, history arena, work
+// buffers, one HIP stream), the RK stage sequencing of RK54 / SSPRK104
+// (runge_kutta.cpp:19-261), the scalar time bookkeeping of appendAll/replaceAll
+// (time_steps.cpp:49-81,127-162) and the adaptive main loop
+// (simulation_runner.cpp:125-304, CPU-only branches).  All N- and N*N-sized
+// arithmetic runs in the gfx950 kernels (dmfe_kernels.hip); the host only
+// replays the reference's scalar arithmetic (t1grid, delta_t_ratio, dt) in the
+// same FP64 order, so that time positions are bit-identical.
+#include "dmfe_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace dmfe {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+namespace {
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return set_error(DMFE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+// Dormand-Prince 5(4) and SSPRK(10,4) tableaux (runge_kutta.cpp:22-35,149-159)
+struct Tableau {
+    int S;
+    double A[kMaxStages][kMaxStages];
+    double b[kMaxStages];
+    double b2[kMaxStages];
+};
+
+const Tableau& dp54() {
+    static Tableau t = [] {
+        Tableau x{};
+        x.S = 7;
+        const double A[7][7] = {{0, 0, 0, 0, 0, 0, 0},
+                                {1.0 / 5, 0, 0, 0, 0, 0, 0},
+                                {3.0 / 40, 9.0 / 40, 0, 0, 0, 0, 0},
+                                {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0, 0},
+                                {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0, 0},
+                                {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0, 0},
+                                {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84, 0}};
+        const double b[7] = {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84, 0};
+        const double b2[7] = {5179.0 / 57600, 0, 7571.0 / 16695, 393.0 / 640, -92097.0 / 339200, 187.0 / 2100, 1.0 / 40};
+        for (int i = 0; i < 7; ++i) {
+            for (int j = 0; j < 7; ++j) x.A[i][j] = A[i][j];
+            x.b[i] = b[i];
+            x.b2[i] = b2[i];
+        }
+        return x;
+    }();
+    return t;
+}
+
+const Tableau& ssprk104() {
+    static Tableau t = [] {
+        Tableau x{};
+        x.S = 10;
+        for (int i = 0; i < 10; ++i) {
+            x.b[i] = 1.0 / 10;
+            for (int j = 0; j < 10; ++j) x.A[i][j] = (j < i) ? ((i <= 4) ? 1.0 / 6 : (j <= 4 ? 1.0 / 15 : 1.0 / 6)) : 0.0;
+        }
+        const double b2[10] = {0., 2.0 / 9, 0, 0, 5.0 / 18, 1.0 / 3, 0., 0., 0., 1.0 / 6};
+        for (int i = 0; i < 10; ++i) x.b2[i] = b2[i];
+        return x;
+    }();
+    return t;
+}
+
+double host_fl(const dmfe_params& p, double q, int d) {
+    auto pw = [](double b, int e) {
+        if (e == 0) return 1.0;
+        if (e == 1) return b;
+        if (e == 2) return b * b;
+        if (e == 3) return b * b * b;
+        if (e == 4) { double s = b * b; return s * s; }
+        double r = 1.0, c = b;
+        while (e > 0) { if (e & 1) r *= c; c *= c; e >>= 1; }
+        return r;
+    };
+    const double l = p.lambda;
+    const int a = p.p, b = p.p2;
+    switch (d) {
+        case 1: return l * a * pw(q, a - 1) + (1 - l) * b * pw(q, b - 1);
+        case 2: return l * a * (a - 1) * pw(q, a - 2) + (1 - l) * b * (b - 1) * pw(q, b - 2);
+        default: return l * pw(q, a) + (1 - l) * pw(q, b);
+    }
+}
+
+}  // namespace
+}  // namespace dmfe
+
+using namespace dmfe;
+
+enum KernelKind { kKindSearch = 0, kKindRhs = 1, kKindFinish = 2, kKindOther = 3, kKinds = 4 };
+
+struct dmfe_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dmfe_params params{};
+    Model model{};
+    int N = 0, m = 0;
+    long long P = 0;
+    int split = 2;
+    // tables
+    double *theta = nullptr, *integ = nullptr, *phi2 = nullptr;
+    int* inds[3] = {nullptr, nullptr, nullptr};
+    double* w[3] = {nullptr, nullptr, nullptr};
+    // history arena
+    int64_t cap = 0, T1 = 0;
+    History H{};
+    std::vector<double> t1, rho;  // host mirrors of t1grid and delta_t_ratio
+    // work
+    Work W{};
+    double* fields = nullptr;
+    double* pinned = nullptr;  // host scalars (pinned)
+    // control (SimulationConfig / RKData scalars)
+    double dt = 0, delta = 0, delta_old = 0, specRad = 0;
+    int64_t loop = 0;
+    int method = DMFE_RK54;
+    bool interp_valid = false;
+    // optional per-kernel timing with HIP events on the context stream
+    bool profiling = false;
+    std::vector<hipEvent_t> ev[kKinds];  // start/stop pairs
+    size_t ev_used[kKinds] = {0, 0, 0, 0};
+
+    Tables tables() const {
+        Tables T{};
+        T.N = N; T.m = m; T.P = P;
+        T.theta = theta; T.integ = integ; T.phi2 = phi2;
+        for (int f = 0; f < 3; ++f) { T.inds[f] = inds[f]; T.w[f] = w[f]; }
+        return T;
+    }
+};
+
+namespace {
+
+void free_history(History& h) {
+    hipFree(h.V); hipFree(h.D); hipFree(h.t1); hipFree(h.rho); hipFree(h.r); hipFree(h.dr);
+    h = History{};
+}
+
+int alloc_history(History& h, int64_t rows, int N) {
+    const size_t n = (size_t)rows;
+    if (hipMalloc(&h.V, n * N * sizeof(double2)) != hipSuccess || hipMalloc(&h.D, n * N * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&h.t1, n * sizeof(double)) != hipSuccess || hipMalloc(&h.rho, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&h.r, n * sizeof(double)) != hipSuccess || hipMalloc(&h.dr, n * sizeof(double)) != hipSuccess) {
+        free_history(h);
+        return set_error(DMFE_ERR_NOMEM, "history allocation of " + std::to_string(rows) + " rows failed");
+    }
+    return DMFE_OK;
+}
+
+// Grows the history arena to hold at least `rows` rows (copying the live rows).
+int ensure_rows(dmfe_ctx* c, int64_t rows) {
+    if (rows <= c->cap) return DMFE_OK;
+    int64_t nc = std::max<int64_t>(c->cap, 64);
+    while (nc < rows) nc *= 2;
+    History h{};
+    int rc = alloc_history(h, nc, c->N);
+    if (rc) return rc;
+    if (c->T1 > 0) {
+        const size_t n = (size_t)c->T1;
+        HIP_TRY(hipMemcpyAsync(h.V, c->H.V, n * c->N * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(h.D, c->H.D, n * c->N * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(h.t1, c->H.t1, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(h.rho, c->H.rho, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(h.r, c->H.r, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(h.dr, c->H.dr, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    free_history(c->H);
+    c->H = h;
+    c->cap = nc;
+    return DMFE_OK;
+}
+
+void prof_begin(dmfe_ctx* c, int kind) {
+    if (!c->profiling) return;
+    auto& v = c->ev[kind];
+    const size_t i = c->ev_used[kind];
+    if (i + 2 > v.size()) {
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        v.push_back(a);
+        v.push_back(b);
+    }
+    hipEventRecord(v[i], c->stream);
+}
+
+void prof_end(dmfe_ctx* c, int kind) {
+    if (!c->profiling) return;
+    hipEventRecord(c->ev[kind][c->ev_used[kind] + 1], c->stream);
+    c->ev_used[kind] += 2;
+}
+
+int launch_check(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return DMFE_OK;
+}
+
+// interpolate(): positions + B1 prologue (interpolation_core.cpp:19-113)
+int run_search(dmfe_ctx* c, int mode, const double* srcB1, const double* srcB2, double* saveB1, double* saveB2) {
+    SearchArgs a{};
+    a.mode = mode;
+    a.T1 = (int)c->T1;
+    a.tl = c->t1[c->T1 - 1];
+    a.srcB1 = srcB1; a.srcB2 = srcB2; a.saveB1 = saveB1; a.saveB2 = saveB2;
+    prof_begin(c, kKindSearch);
+    launch_search(c->tables(), c->H, c->W, c->model, a, c->stream);
+    prof_end(c, kKindSearch);
+    c->interp_valid = true;
+    return launch_check("search_kernel");
+}
+
+int run_rhs_rows(dmfe_ctx* c, double* fields) {
+    RhsArgs a{};
+    a.T1 = (int)c->T1;
+    a.tl = c->t1[c->T1 - 1];
+    a.split = c->split;
+    a.log_response = c->params.log_response;
+    a.fields = fields;
+    prof_begin(c, kKindRhs);
+    launch_rhs_rows(c->tables(), c->H, c->W, c->model, a, c->stream);
+    prof_end(c, kKindRhs);
+    return launch_check("rhs_rows_kernel");
+}
+
+int copy_out(dmfe_ctx* c, double* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMFE_OK;
+}
+
+// update() = RK54() / SSPRK104() (runge_kutta.cpp:19-261).
+int rk_step(dmfe_ctx* c, double* err_out) {
+    const bool ss = (c->method != DMFE_RK54);
+    const Tableau& tb = ss ? ssprk104() : dp54();
+    const int S = tb.S;
+    int rc = ensure_rows(c, c->T1 + 1);
+    if (rc) return rc;
+    if (!c->interp_valid && c->T1 > 1) {
+        rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    const Tables T = c->tables();
+    const int64_t g0row = c->T1 - 1;
+    const double dt = c->dt;
+    double gt[kMaxStages + 1];
+    double ht[kMaxStages];
+    gt[0] = c->t1[c->T1 - 1];
+    for (int n = 0; n < S; ++n) {
+        if (c->T1 == 1 || n != 0) {
+            if (n == 0) rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+            else if (ss && n >= 5 && n <= 7)
+                rc = run_search(c, kSearchCopy, c->W.saveB1 + (size_t)(n - 5) * c->N, c->W.saveB2 + (size_t)(n - 5) * c->P,
+                                nullptr, nullptr);
+            else if (ss && n >= 2 && n <= 4)
+                rc = run_search(c, kSearchI, nullptr, nullptr, c->W.saveB1 + (size_t)(n - 2) * c->N,
+                                c->W.saveB2 + (size_t)(n - 2) * c->P);
+            else rc = run_search(c, kSearchI, nullptr, nullptr, nullptr, nullptr);
+            if (rc) return rc;
+        } else if (ss && n >= 2 && n <= 4) {
+            // unreachable (n == 0); kept for symmetry with the reference's stage table
+        }
+        rc = run_rhs_rows(c, nullptr);
+        if (rc) return rc;
+        ht[n] = 1.0;
+        FinishArgs f{};
+        f.mode = 1;
+        f.stage = n;
+        f.split = c->split;
+        f.lr = (int)(c->T1 - 1);
+        f.g0row = (int)g0row;
+        f.t_rhs = c->t1[c->T1 - 1];
+        const bool last = (n == S - 1);
+        f.last = last ? 1 : 0;
+        if (n == 0) {
+            f.nterms = 1;
+            f.coef[0] = dt * tb.A[1][0];
+            gt[1] = gt[0] + dt * tb.A[1][0] * ht[0];
+            // appendAll (time_steps.cpp:140-149)
+            c->t1.push_back(gt[1]);
+            const size_t idx = c->t1.size() - 1;
+            const double tdiff = c->t1[idx] - c->t1[idx - 1];
+            const double r = (idx > 1) ? tdiff / (c->t1[idx - 1] - c->t1[idx - 2]) : 0.0;
+            c->rho.push_back(r);
+            f.wrow = (int)idx;
+            f.t_new = gt[1];
+            f.tdiff = tdiff;
+            f.rho_new = r;
+            c->T1 += 1;
+        } else {
+            gt[n + 1] = gt[0];
+            const int nt = last ? S : n + 1;
+            f.nterms = nt;
+            for (int j = 0; j < nt; ++j) {
+                const double co = last ? tb.b[j] : tb.A[n + 1][j];
+                f.coef[j] = dt * co;
+                gt[n + 1] += dt * co * ht[j];
+            }
+            // replaceAll (time_steps.cpp:57-67)
+            const size_t L = c->t1.size();
+            c->t1[L - 1] = gt[n + 1];
+            const double tdiff = c->t1[L - 1] - c->t1[L - 2];
+            const double r = (L > 2) ? tdiff / (c->t1[L - 2] - c->t1[L - 3]) : 0.0;
+            c->rho[L - 1] = r;
+            f.wrow = (int)(L - 1);
+            f.t_new = gt[n + 1];
+            f.tdiff = tdiff;
+            f.rho_new = r;
+        }
+        if (last) {
+            double gte = gt[0];
+            for (int j = 0; j < S; ++j) {
+                f.coefE[j] = dt * tb.b2[j];
+                gte += dt * tb.b2[j] * ht[j];
+            }
+            f.gt_err = std::fabs(gt[S] - gte);
+        }
+        prof_begin(c, kKindFinish);
+        launch_finish(T, c->H, c->W, c->model, f, c->stream);
+        prof_end(c, kKindFinish);
+        rc = launch_check("finish_kernel");
+        if (rc) return rc;
+    }
+    // final interpolation: RK54 reuses the positions, SSPRK104 searches (:121, :229)
+    rc = run_search(c, ss ? kSearchI : kSearchSame, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->pinned, c->W.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err_out) *err_out = c->pinned[0];
+    return DMFE_OK;
+}
+
+int sparsify(dmfe_ctx* c, double threshold) {
+    const int64_t T1 = c->T1;
+    std::vector<int> keep;
+    keep.reserve((size_t)T1);
+    keep.push_back(0);
+    bool erased = false;
+    if (T1 >= 4) {
+        double* dval = nullptr;
+        HIP_TRY(hipMalloc(&dval, (size_t)T1 * sizeof(double)));
+        launch_sparsify_values(c->tables(), c->H, (int)T1, dval, c->stream);
+        std::vector<double> val((size_t)T1, 0.0);
+        hipError_t e = hipMemcpyAsync(val.data(), dval, (size_t)T1 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(dval);
+        if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("sparsify: ") + hipGetErrorString(e));
+        for (int64_t i = 2; i + 1 < T1; ++i) {
+            if (val[(size_t)i] < threshold) erased = true;
+            else keep.push_back((int)i);
+        }
+    }
+    keep.push_back((int)(T1 - 1));
+    if (!erased) return DMFE_OK;
+    const int nk = (int)keep.size();
+    History h{};
+    int rc = alloc_history(h, c->cap, c->N);
+    if (rc) return rc;
+    int* dkeep = nullptr;
+    HIP_TRY(hipMalloc(&dkeep, (size_t)nk * sizeof(int)));
+    HIP_TRY(hipMemcpyAsync(dkeep, keep.data(), (size_t)nk * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    launch_gather_rows(c->tables(), c->H, h, dkeep, nk, c->stream);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    hipFree(dkeep);
+    free_history(c->H);
+    c->H = h;
+    std::vector<double> nt1(nk), nrho(nk);
+    for (int r = 0; r < nk; ++r) {
+        nt1[r] = c->t1[(size_t)keep[r]];
+        nrho[r] = c->rho[(size_t)keep[r]];
+    }
+    c->t1.swap(nt1);
+    c->rho.swap(nrho);
+    c->T1 = nk;
+    c->interp_valid = false;
+    return DMFE_OK;
+}
+
+// one iteration of runSimulation's while-loop body (simulation_runner.cpp:128-278, CPU-only branches)
+int loop_iteration(dmfe_ctx* c, double* t_out) {
+    const double t = c->t1[c->T1 - 1] + c->dt;
+    c->delta_old = c->delta;
+    double err = 0.0;
+    int rc = rk_step(c, &err);
+    if (rc) return rc;
+    c->delta = err;
+    c->loop++;
+    const double dmax = c->params.delta_max;
+    if (c->loop % 100000 == 0) {
+        int64_t prev = c->T1;
+        int count = 0;
+        while (count < 1) {
+            rc = sparsify(c, dmax);
+            if (rc) return rc;
+            if (c->T1 >= prev) break;
+            prev = c->T1;
+            count++;
+        }
+        rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+        // last_rollback_loop stays -1000 on the CPU path (simulation_runner.cpp:53,220)
+        if (c->delta < dmax / 2 && c->loop - (-1000) > 1000) {
+            c->dt *= 0.5;
+            c->method = DMFE_SSPRK104;
+        }
+    }
+    const double* rmax = c->params.rmax;
+    if (c->delta < dmax && c->loop > 5 && (c->delta < 1.1 * c->delta_old || c->delta_old < dmax / 1000) &&
+        rmax[c->method - 1] / c->specRad > c->dt && c->rho.back() == 1.0) {
+        c->dt *= 1.01;
+    } else if (c->delta > 2 * dmax && c->dt > c->params.delta_t_min) {
+        c->dt *= 0.9;
+    }
+    if (c->method == DMFE_SSPRK104 && c->delta > dmax && rmax[0] / c->specRad > c->dt) {
+        c->method = DMFE_RK54;
+        c->dt *= 0.5;
+    }
+    if (t_out) *t_out = t;
+    return DMFE_OK;
+}
+
+int observables(dmfe_ctx* c, double t, double* E, double* qk0) {
+    launch_row_scalar(c->tables(), c->H, c->W, c->model, (int)(c->T1 - 1), t, 1, c->stream);
+    int rc = launch_check("row_scalar_kernel");
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->pinned, c->W.scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->pinned + 1, &c->H.V[(size_t)(c->T1 - 1) * c->N], sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (E) *E = c->pinned[0];
+    if (qk0) *qk0 = c->pinned[1];
+    return DMFE_OK;
+}
+
+void free_ctx(dmfe_ctx* c) {
+    if (!c) return;
+    hipFree(c->theta); hipFree(c->integ); hipFree(c->phi2);
+    for (int f = 0; f < 3; ++f) { hipFree(c->inds[f]); hipFree(c->w[f]); }
+    free_history(c->H);
+    hipFree(c->W.posB1); hipFree(c->W.posB2); hipFree(c->W.saveB1); hipFree(c->W.saveB2);
+    hipFree(c->W.Floor); hipFree(c->W.diff); hipFree(c->W.rInt); hipFree(c->W.partial);
+    hipFree(c->W.qkb1c0); hipFree(c->W.hK); hipFree(c->W.hR); hipFree(c->W.scal);
+    hipFree(c->fields);
+    if (c->pinned) hipHostFree(c->pinned);
+    for (auto& v : c->ev)
+        for (hipEvent_t e : v) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // namespace
+
+// ============================== C ABI ==============================================
+
+extern "C" const char* dmfe_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char* dmfe_version(void) { return "dmfe-mi355x 0.1.0"; }
+
+extern "C" void dmfe_params_default(dmfe_params* p) {
+    if (!p) return;
+    *p = dmfe_params{};
+    p->p = 3; p->p2 = 12; p->lambda = 0.3; p->T0 = 1e50; p->Gamma = 0.0;
+    p->delta_t_min = 1e-5; p->delta_max = 1e-10;
+    const double r[11] = {3, 13, 20, 80, 180, 320, 500, 720, 980, 1280, 1620};
+    std::memcpy(p->rmax, r, sizeof r);
+    p->log_response = 0;
+}
+
+extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64_t rows_hint, dmfe_ctx** out) {
+    if (!g || !prm || !out) return set_error(DMFE_ERR_ARG, "dmfe_ctx_create: null argument");
+    *out = nullptr;
+    if (g->N < 4 || g->N > kMaxN) return set_error(DMFE_ERR_ARG, "dmfe_ctx_create: N must be in [4, 2048]");
+    if (g->m < 1 || g->m > g->N) return set_error(DMFE_ERR_ARG, "dmfe_ctx_create: stencil width m must be in [1, N]");
+    if (!g->theta || !g->integ || !g->phi2) return set_error(DMFE_ERR_ARG, "dmfe_ctx_create: missing theta/int/phi2");
+    for (int f = 0; f < 3; ++f)
+        if (!g->inds[f] || !g->weights[f]) return set_error(DMFE_ERR_ARG, "dmfe_ctx_create: missing stencil table");
+    const int N = (int)g->N, m = (int)g->m;
+    const long long P = (long long)N * N;
+    // validate stencil windows: start in [0, N-m] (negative = dense row starting at 0)
+    for (int f = 0; f < 3; ++f)
+        for (long long k = 0; k < P; ++k) {
+            const int64_t s = g->inds[f][k] < 0 ? 0 : g->inds[f][k];
+            if (s + m > N)
+                return set_error(DMFE_ERR_ARG, "dmfe_ctx_create: stencil start " + std::to_string(g->inds[f][k]) +
+                                                   " + width exceeds N in table " + std::to_string(f));
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(DMFE_ERR_HIP, "dmfe_ctx_create: no HIP device");
+    dmfe_ctx* c = new dmfe_ctx();
+    hipGetDevice(&c->device);
+    c->params = *prm;
+    c->model.lambda = prm->lambda; c->model.T0 = prm->T0; c->model.Gamma = prm->Gamma;
+    c->model.p = prm->p; c->model.p2 = prm->p2;
+    c->N = N; c->m = m; c->P = P;
+    c->split = (N >= 256) ? std::max(1, N / 256) : 1;
+    c->specRad = 4 * std::sqrt(host_fl(*prm, 1, 2));
+    auto fail = [&](int code, const std::string& msg) { free_ctx(c); return set_error(code, msg); };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(DMFE_ERR_HIP, "stream create failed");
+    const size_t bN = (size_t)N * sizeof(double), bP = (size_t)P * sizeof(double);
+    bool ok = hipMalloc(&c->theta, bN) == hipSuccess && hipMalloc(&c->integ, bN) == hipSuccess &&
+              hipMalloc(&c->phi2, bP) == hipSuccess;
+    for (int f = 0; ok && f < 3; ++f)
+        ok = hipMalloc(&c->inds[f], (size_t)P * sizeof(int)) == hipSuccess &&
+             hipMalloc(&c->w[f], bP * (size_t)m) == hipSuccess;
+    ok = ok && hipMalloc(&c->W.posB1, bN) == hipSuccess && hipMalloc(&c->W.posB2, bP) == hipSuccess &&
+         hipMalloc(&c->W.saveB1, 3 * bN) == hipSuccess && hipMalloc(&c->W.saveB2, 3 * bP) == hipSuccess &&
+         hipMalloc(&c->W.Floor, (size_t)N * sizeof(int)) == hipSuccess && hipMalloc(&c->W.diff, bN) == hipSuccess &&
+         hipMalloc(&c->W.rInt, bN) == hipSuccess &&
+         hipMalloc(&c->W.partial, bN * (size_t)c->split * kSums) == hipSuccess &&
+         hipMalloc(&c->W.qkb1c0, bN) == hipSuccess && hipMalloc(&c->W.hK, bN * kMaxStages) == hipSuccess &&
+         hipMalloc(&c->W.hR, bN * kMaxStages) == hipSuccess && hipMalloc(&c->W.scal, 8 * sizeof(double)) == hipSuccess &&
+         hipHostMalloc((void**)&c->pinned, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    if (!ok) return fail(DMFE_ERR_NOMEM, "dmfe_ctx_create: device allocation failed");
+    // uploads: weights transposed [P][m] -> [m][P], stencil starts to int32
+    std::vector<double> wt((size_t)P * m);
+    std::vector<int> it((size_t)P);
+    bool up = hipMemcpy(c->theta, g->theta, bN, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->integ, g->integ, bN, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->phi2, g->phi2, bP, hipMemcpyHostToDevice) == hipSuccess;
+    for (int f = 0; up && f < 3; ++f) {
+        for (long long k = 0; k < P; ++k) {
+            it[(size_t)k] = g->inds[f][k] < 0 ? 0 : (int)g->inds[f][k];
+            for (int d = 0; d < m; ++d) wt[(size_t)d * P + k] = g->weights[f][(size_t)k * m + d];
+        }
+        up = hipMemcpy(c->inds[f], it.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(c->w[f], wt.data(), bP * (size_t)m, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    // posB1xOld = 1, posB2xOld = 0 (initialization.cpp:234-235)
+    std::vector<double> ones((size_t)N, 1.0), zeros((size_t)P, 0.0);
+    up = up && hipMemcpy(c->W.posB1, ones.data(), bN, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->W.posB2, zeros.data(), bP, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(c->W.scal, 0, 8 * sizeof(double)) == hipSuccess;
+    if (!up) return fail(DMFE_ERR_HIP, "dmfe_ctx_create: upload failed");
+    int rc = ensure_rows(c, std::max<int64_t>(rows_hint, 64));
+    if (rc) { std::string msg = g_err; free_ctx(c); return set_error(rc, msg); }
+    *out = c;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_ctx_destroy(dmfe_ctx* c) {
+    if (c) hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_init_quench(dmfe_ctx* c) {
+    if (!c) return set_error(DMFE_ERR_ARG, "null context");
+    const int N = c->N;
+    int rc = ensure_rows(c, 1);
+    if (rc) return rc;
+    c->T1 = 1;
+    c->t1.assign(1, 0.0);
+    c->rho.assign(1, 0.0);
+    c->specRad = 4 * std::sqrt(host_fl(c->params, 1, 2));
+    c->dt = c->params.delta_t_min;
+    c->loop = 0;
+    c->delta = 1;
+    c->delta_old = 0;
+    std::vector<double2> v((size_t)N, make_double2(1.0, 1.0)), d((size_t)N, make_double2(0.0, 0.0));
+    const double zero = 0.0;
+    const double r0 = c->params.Gamma + host_fl(c->params, 1, 1) / c->params.T0;
+    HIP_TRY(hipMemcpyAsync(c->H.V, v.data(), N * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.D, d.data(), N * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.t1, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.rho, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.r, &r0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // drvec[0] = rstep() on the initial row (initialization.cpp:222)
+    launch_row_scalar(c->tables(), c->H, c->W, c->model, 0, 0.0, 0, c->stream);
+    rc = launch_check("row_scalar_kernel");
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->H.dr, c->W.scal + 2, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    c->method = (c->dt < c->params.rmax[0] / c->specRad) ? DMFE_RK54 : DMFE_SSPRK104;
+    rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_get_T1(const dmfe_ctx* c, int64_t* T1) {
+    if (!c || !T1) return set_error(DMFE_ERR_ARG, "null argument");
+    *T1 = c->T1;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_set_state(dmfe_ctx* c, int64_t T1, const double* t1grid, const double* QK, const double* QR,
+                              const double* dQK, const double* dQR, const double* rvec, const double* drvec,
+                              const double* dtratio) {
+    if (!c || T1 < 1 || !t1grid || !QK || !QR || !dQK || !dQR || !rvec || !drvec)
+        return set_error(DMFE_ERR_ARG, "dmfe_set_state: null argument or T1 < 1");
+    for (int64_t i = 1; i < T1; ++i)
+        if (!(t1grid[i] > t1grid[i - 1])) return set_error(DMFE_ERR_ARG, "dmfe_set_state: t1grid must be strictly increasing");
+    int rc = ensure_rows(c, T1 + 1);
+    if (rc) return rc;
+    const size_t N = (size_t)c->N, n = (size_t)T1;
+    c->T1 = T1;
+    c->t1.assign(t1grid, t1grid + n);
+    c->rho.assign(n, 0.0);
+    if (dtratio) {
+        c->rho.assign(dtratio, dtratio + n);
+    } else {  // io_input.cpp:169-172
+        for (size_t i = 2; i < n; ++i) c->rho[i] = (c->t1[i] - c->t1[i - 1]) / (c->t1[i - 1] - c->t1[i - 2]);
+    }
+    std::vector<double2> v(n * N), d(n * N);
+    for (size_t k = 0; k < n * N; ++k) { v[k] = make_double2(QK[k], QR[k]); d[k] = make_double2(dQK[k], dQR[k]); }
+    HIP_TRY(hipMemcpyAsync(c->H.V, v.data(), n * N * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.D, d.data(), n * N * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.t1, c->t1.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.rho, c->rho.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.r, rvec, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->H.dr, drvec, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->interp_valid = false;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_get_state(dmfe_ctx* c, double* t1grid, double* QK, double* QR, double* dQK, double* dQR,
+                              double* rvec, double* drvec, double* dtratio) {
+    if (!c) return set_error(DMFE_ERR_ARG, "null context");
+    const size_t N = (size_t)c->N, n = (size_t)c->T1;
+    if (t1grid) std::memcpy(t1grid, c->t1.data(), n * sizeof(double));
+    if (dtratio) std::memcpy(dtratio, c->rho.data(), n * sizeof(double));
+    if (QK || QR || dQK || dQR) {
+        std::vector<double2> v(n * N), d(n * N);
+        HIP_TRY(hipMemcpyAsync(v.data(), c->H.V, n * N * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(d.data(), c->H.D, n * N * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (size_t k = 0; k < n * N; ++k) {
+            if (QK) QK[k] = v[k].x;
+            if (QR) QR[k] = v[k].y;
+            if (dQK) dQK[k] = d[k].x;
+            if (dQR) dQR[k] = d[k].y;
+        }
+    }
+    if (rvec) { int rc = copy_out(c, rvec, c->H.r, n * sizeof(double)); if (rc) return rc; }
+    if (drvec) { int rc = copy_out(c, drvec, c->H.dr, n * sizeof(double)); if (rc) return rc; }
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_set_control(dmfe_ctx* c, double delta_t, double delta, double delta_old, int64_t loop, int method) {
+    if (!c || (method != DMFE_RK54 && method != DMFE_SSPRK104)) return set_error(DMFE_ERR_ARG, "dmfe_set_control: bad argument");
+    c->dt = delta_t; c->delta = delta; c->delta_old = delta_old; c->loop = loop; c->method = method;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_get_control(const dmfe_ctx* c, double* delta_t, double* delta, double* delta_old, int64_t* loop,
+                                int* method) {
+    if (!c) return set_error(DMFE_ERR_ARG, "null context");
+    if (delta_t) *delta_t = c->dt;
+    if (delta) *delta = c->delta;
+    if (delta_old) *delta_old = c->delta_old;
+    if (loop) *loop = c->loop;
+    if (method) *method = c->method;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_interpolate(dmfe_ctx* c, int mode) {
+    if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_interpolate: no state");
+    if (mode != DMFE_POS_BSEARCH && mode != DMFE_POS_ISEARCH && mode != DMFE_POS_SAME)
+        return set_error(DMFE_ERR_ARG, "dmfe_interpolate: bad mode");
+    const int km = mode == DMFE_POS_BSEARCH ? kSearchB : (mode == DMFE_POS_ISEARCH ? kSearchI : kSearchSame);
+    int rc = run_search(c, km, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_get_positions(dmfe_ctx* c, double* posB1, double* posB2) {
+    if (!c) return set_error(DMFE_ERR_ARG, "null context");
+    if (posB1) { int rc = copy_out(c, posB1, c->W.posB1, (size_t)c->N * sizeof(double)); if (rc) return rc; }
+    if (posB2) { int rc = copy_out(c, posB2, c->W.posB2, (size_t)c->P * sizeof(double)); if (rc) return rc; }
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_set_positions(dmfe_ctx* c, const double* posB1, const double* posB2) {
+    if (!c || !posB1 || !posB2) return set_error(DMFE_ERR_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(c->W.posB1, posB1, (size_t)c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->W.posB2, posB2, (size_t)c->P * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_get_fields(dmfe_ctx* c, double* out) {
+    if (!c || !out || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_get_fields: bad argument");
+    if (!c->fields) HIP_TRY(hipMalloc(&c->fields, (size_t)c->P * 15 * sizeof(double)));
+    int rc = run_rhs_rows(c, c->fields);
+    if (rc) return rc;
+    rc = copy_out(c, out, c->fields, (size_t)c->P * 15 * sizeof(double));
+    if (rc) return rc;
+    return copy_out(c, out + (size_t)c->P * 15, c->W.rInt, (size_t)c->N * sizeof(double));
+}
+
+extern "C" int dmfe_rhs(dmfe_ctx* c, double* hK, double* hR) {
+    if (!c || !hK || !hR || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_rhs: bad argument");
+    int rc = run_rhs_rows(c, nullptr);
+    if (rc) return rc;
+    FinishArgs f{};
+    f.mode = 0;
+    f.stage = 0;
+    f.split = c->split;
+    f.lr = (int)(c->T1 - 1);
+    f.t_rhs = c->t1[c->T1 - 1];
+    launch_finish(c->tables(), c->H, c->W, c->model, f, c->stream);
+    rc = launch_check("finish_kernel");
+    if (rc) return rc;
+    rc = copy_out(c, hK, c->W.hK, (size_t)c->N * sizeof(double));
+    if (rc) return rc;
+    return copy_out(c, hR, c->W.hR, (size_t)c->N * sizeof(double));
+}
+
+extern "C" int dmfe_rstep(dmfe_ctx* c, double* r) {
+    if (!c || !r || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_rstep: bad argument");
+    launch_row_scalar(c->tables(), c->H, c->W, c->model, (int)(c->T1 - 1), c->t1[c->T1 - 1], 0, c->stream);
+    int rc = launch_check("row_scalar_kernel");
+    if (rc) return rc;
+    return copy_out(c, r, c->W.scal + 2, sizeof(double));
+}
+
+extern "C" int dmfe_rk_step(dmfe_ctx* c, double* err) {
+    if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_rk_step: no state");
+    return rk_step(c, err);
+}
+
+extern "C" int dmfe_loop_iteration(dmfe_ctx* c, double* t_out) {
+    if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_loop_iteration: no state");
+    return loop_iteration(c, t_out);
+}
+
+extern "C" int dmfe_run(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps) {
+    if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_run: no state");
+    int64_t done = 0;
+    double t = c->t1[c->T1 - 1];
+    const double eps = std::numeric_limits<double>::epsilon();
+    while (t < tmax && c->loop < max_loop &&
+           c->dt >= c->params.delta_t_min - eps * std::max(std::fabs(c->dt), std::fabs(c->params.delta_t_min))) {
+        int rc = loop_iteration(c, &t);
+        if (rc) { if (steps) *steps = done; return rc; }
+        if (corr && done < corr_cap) {
+            double E = 0, q0 = 0;
+            rc = observables(c, t, &E, &q0);
+            if (rc) { if (steps) *steps = done; return rc; }
+            corr[3 * done + 0] = t;
+            corr[3 * done + 1] = E;
+            corr[3 * done + 2] = q0;
+        }
+        ++done;
+    }
+    if (steps) *steps = done;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_observables(dmfe_ctx* c, double t, double* energy, double* qk0) {
+    if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_observables: no state");
+    return observables(c, t, energy, qk0);
+}
+
+extern "C" int dmfe_sparsify(dmfe_ctx* c, double threshold) {
+    if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_sparsify: no state");
+    return sparsify(c, threshold);
+}
+
+extern "C" int dmfe_synchronize(dmfe_ctx* c) {
+    if (!c) return set_error(DMFE_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_profile(dmfe_ctx* c, int enable) {
+    if (!c) return set_error(DMFE_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->profiling = enable != 0;
+    for (int k = 0; k < kKinds; ++k) c->ev_used[k] = 0;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_profile_read(dmfe_ctx* c, int kind, double* total_ms, int64_t* launches) {
+    if (!c || kind < 0 || kind >= kKinds || !total_ms || !launches)
+        return set_error(DMFE_ERR_ARG, "dmfe_profile_read: bad argument");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    const size_t n = c->ev_used[kind];
+    for (size_t i = 0; i + 1 < n; i += 2) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[kind][i], c->ev[kind][i + 1]));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int64_t)(n / 2);
+    return DMFE_OK;
+}

@@ -1,0 +1,123 @@
+// Internal declarations shared by the host library (dmfe_ctx.cpp, dmfe_grid.cpp)
+// and the gfx950 kernels (dmfe_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/dmfe.h"
+
+namespace dmfe {
+
+int set_error(int code, const std::string& msg);
+
+// Grid generator pieces (dmfe_grid.cpp)
+void grid_theta(std::size_t len, double Tmax, double alpha, double delta, std::vector<long double>& theta);
+void grid_quadrature(const std::vector<long double>& x, int order, std::vector<long double>& w);
+void grid_poly_stencils(const std::vector<long double>& x, const std::vector<long double>& q, int n,
+                        int64_t* starts, double* alpha);
+void grid_rational_stencils(const std::vector<long double>& x, const std::vector<long double>& q, int d, int m,
+                            int64_t* starts, double* alpha);
+void grid_bspline_weights(const std::vector<long double>& x, const std::vector<long double>& q, int p, double* W);
+
+constexpr int kMaxN = 2048;     // LDS sizing of the row kernels
+constexpr int kMaxStages = 10;  // SSPRK(10,4)
+constexpr int kSums = 8;        // quadratures per RHS row
+
+// f(q) = lambda q^p + (1-lambda) q^p2 and its derivatives (math_ops.cpp:23-37)
+struct Model {
+    double lambda, T0, Gamma;
+    int p, p2;
+};
+
+struct Tables {
+    int N, m;
+    long long P;
+    const double* theta;
+    const double* integ;
+    const double* phi2;
+    const int* inds[3];    // stencil starts A1, A2, B2 [P]
+    const double* w[3];    // weights A1, A2, B2, transposed to [m][P]
+};
+
+struct History {
+    double2* V;   // (QK, QR)   [cap][N]
+    double2* D;   // (dQK, dQR) [cap][N]
+    double* t1;   // t1grid
+    double* rho;  // delta_t_ratio
+    double* r;    // rvec
+    double* dr;   // drvec
+};
+
+struct Work {
+    double* posB1;    // [N]  posB1xOld
+    double* posB2;    // [P]  posB2xOld
+    double* saveB1;   // [3][N] SSPRK stage positions
+    double* saveB2;   // [3][P]
+    int* Floor;       // [N]  B1 time rows
+    double* diff;     // [N]  Floor - posB1
+    double* rInt;     // [N]
+    double* partial;  // [N][split][kSums]
+    double* qkb1c0;   // [N]  QKB1[j, 0]
+    double* hK;       // [kMaxStages][N]
+    double* hR;       // [kMaxStages][N]
+    double* scal;     // [0] dr (drstep2), [1] error norm, [2] rstep / energy scratch
+};
+
+enum SearchMode { kSearchB = 0, kSearchI = 1, kSearchSame = 2, kSearchCopy = 3 };
+
+struct SearchArgs {
+    int mode;
+    int T1;
+    double tl;            // t1grid.back()
+    const double* srcB1;  // kSearchCopy source
+    const double* srcB2;
+    double* saveB1;       // optional second destination
+    double* saveB2;
+};
+
+struct RhsArgs {
+    int T1;
+    double tl;        // t1grid.back() used by the RHS
+    int split;        // column chunks per row
+    int log_response;
+    double* fields;   // non-null: materialise the 15 fields (debug)
+};
+
+struct FinishArgs {
+    int mode;         // 0: RHS only (into hK[stage]); 1: full RK stage
+    int stage;
+    int nterms;       // h_0..h_{nterms-1} combine into g_{stage+1}
+    int split;
+    int lr;           // current last history row (RHS state)
+    int g0row;        // step start row
+    int wrow;         // row written (append: lr + 1, replace: lr)
+    double t_rhs;     // t of the RHS
+    double t_new;     // gt[stage + 1]
+    double tdiff;
+    double rho_new;
+    double coef[kMaxStages];   // dt * a_{stage+1, j} (or dt * b_j at the last stage)
+    int last;                  // last stage: also the error norm
+    double coefE[kMaxStages];  // dt * b2_j
+    double gt_err;             // |gt_S - gte|
+};
+
+void launch_search(const Tables& T, const History& H, const Work& W, const Model& M, const SearchArgs& a,
+                   hipStream_t s);
+void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
+                     hipStream_t s);
+void launch_finish(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
+                   hipStream_t s);
+// rstep (what = 0) or energy at time t (what = 1) on history row `row`, into W.scal[2]
+void launch_row_scalar(const Tables& T, const History& H, const Work& W, const Model& M, int row, double t,
+                       int what, hipStream_t s);
+// sparsify: per-row values val[i] for i in [2, T1-2]
+void launch_sparsify_values(const Tables& T, const History& H, int T1, double* val, hipStream_t s);
+// gather kept rows: dst row r <- src row keep[r]
+void launch_gather_rows(const Tables& T, const History& src, const History& dst, const int* keep, int nkeep,
+                        hipStream_t s);
+
+}  // namespace dmfe

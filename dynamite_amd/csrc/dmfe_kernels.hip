@@ -1,0 +1,688 @@
+// gfx950 kernels of the per-RK-stage DMFT update (FP64).
+//
+//   search_kernel    time positions of theta*t (B1) and phi2*t (B2) - binary search or
+//                    the seeded interpolation search, bit-exact with search_utils.cpp:13-131 -
+//                    and the B1 prologue (Floor, diff, rInt; interpolation_core.cpp:57-113).
+//   rhs_rows_kernel  one (row j, column chunk) per workgroup: rebuilds the A1/A2/B1/B2
+//                    interpolants of row j (index_vec.cpp:15-156, index_mat.cpp:18-91), the
+//                    self-energies (math_sigma.cpp:7-62) and the eight int.dat quadratures
+//                    of QKstep/QRstep (time_steps.cpp:27-47,94-100) without materialising
+//                    any N*N field in HBM.
+//   finish_kernel    one workgroup: RHS epilogue, RK stage combination (runge_kutta.cpp),
+//                    append/replace of the history row (time_steps.cpp:49-81,127-162),
+//                    rstep, drstep2 and the embedded error norm.
+//
+// Memory layout (HBM): history as two row-major [T1][N] arrays of double2,
+// V = (QK, QR) and D = (dQK, dQR), so every gather is one 16-byte load that
+// serves both fields; stencil weights transposed to [m][N*N] so a wavefront's
+// 64 lanes (64 consecutive columns of one row) read 512 contiguous bytes per tap.
+#include "dmfe_internal.h"
+
+namespace dmfe {
+namespace {
+
+constexpr int kWave = 64;
+
+// ---- f and derivatives (math_ops.cpp:7-37) ----------------------------------------
+__device__ inline double pow_int(double b, int e) {
+    if (e == 0) return 1.0;
+    if (e == 1) return b;
+    if (e == 2) return b * b;
+    if (e == 3) return b * b * b;
+    if (e == 4) { const double sq = b * b; return sq * sq; }
+    double r = 1.0, c = b;
+    while (e > 0) {
+        if (e & 1) r *= c;
+        c *= c;
+        e >>= 1;
+    }
+    return r;
+}
+__device__ inline double fl1(const Model& M, double q) {
+    return M.lambda * M.p * pow_int(q, M.p - 1) + (1 - M.lambda) * M.p2 * pow_int(q, M.p2 - 1);
+}
+__device__ inline double fl2(const Model& M, double q) {
+    return M.lambda * M.p * (M.p - 1) * pow_int(q, M.p - 2) + (1 - M.lambda) * M.p2 * (M.p2 - 1) * pow_int(q, M.p2 - 2);
+}
+__device__ inline double fl3(const Model& M, double q) {
+    return M.lambda * M.p * (M.p - 1) * (M.p - 2) * pow_int(q, M.p - 3) +
+           (1 - M.lambda) * M.p2 * (M.p2 - 1) * (M.p2 - 2) * pow_int(q, M.p2 - 3);
+}
+
+// ---- reductions -------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Sums K values over the block; every thread gets the totals in out[]. Fixed order.
+template <int K, int NT>
+__device__ inline void block_sum(double (&v)[K], double* lds /* K*NT/64 */) {
+    constexpr int NW = NT / kWave;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) lds[k * NW + wid] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < NW; ++w) s += lds[k * NW + w];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+// ---- searches (search_utils.cpp:13-131) --------------------------------------------
+// Every list index is clamped into [0, length) so that no input can read out of
+// bounds; on the reference's own (well-formed) inputs the clamp never binds.
+#pragma clang fp contract(off)
+__device__ inline double list_at(const double* list, size_t length, size_t i) {
+    return list[i < length ? i : length - 1];
+}
+
+__device__ double bsearch_elem(const double* list, size_t len, double El) {
+    size_t n0 = 1, n1 = len, m = 1;
+    double Lm = 0.0;
+    if (list[0] < El) {
+        while (n0 <= n1) {
+            m = (n0 + n1) / 2;
+            Lm = list_at(list, len, m - 1);
+            if (Lm == El) { n0 = m; break; }
+            if (Lm < El) n0 = m + 1;
+            else n1 = m - 1;
+        }
+    }
+    if (Lm <= El) {
+        if (m == len) return (double)m;
+        return m + (El - Lm) / (list_at(list, len, m) - Lm);
+    }
+    if (m > 1) return m - (El - Lm) / (list_at(list, len, m - 2) - Lm);
+    return (double)m;
+}
+
+// One element of isearchPosSortedInit with its upper bracket n1 given.
+__device__ double isearch_elem(const double* list, size_t length, double El, double init, size_t n1) {
+    size_t n0 = (size_t)floor(init);
+    if (n0 < 1) n0 = 1;  // the reference reads list[-1] here; positions are >= 1 on its inputs
+    size_t m = (n0 + 1 < length) ? n0 + 1 : length;
+    bool even = true;
+    double l0, l1, Lm = list_at(list, length, m - 1);
+    if (Lm > El) {
+        const size_t a = m - 2;  // wraps for m < 2 exactly like the size_t code
+        n1 = a > 1 ? a : 1;
+    }
+    int guard = 0;
+    while ((l0 = list_at(list, length, n0 - 1)) <= El && El <= (l1 = list_at(list, length, n1 - 1)) && n0 < n1 &&
+           guard++ < 512) {
+        even = !even;
+        if (even) m = (size_t)((double)n0 + round((El - l0) / (l1 - l0) * (double)(n1 - n0)));
+        else m = (n0 + n1) / 2;
+        Lm = list_at(list, length, m - 1);
+        if (Lm == El) { n0 = m; n1 = m - 1; }
+        else if (Lm < El) n0 = m + 1;
+        else n1 = m - 1;
+    }
+    if (Lm <= El) {
+        if (m == length) return (double)m;
+        return m + (El - Lm) / (list_at(list, length, m) - Lm);
+    }
+    if (m > 1) return m - (El - Lm) / (list_at(list, length, m - 2) - Lm);
+    return (double)m;
+}
+#pragma clang fp contract(on)
+
+constexpr int kSearchThreads = 256;
+constexpr int kSearchEPT = kMaxN / kSearchThreads;
+
+// Positions of one row of `len` elements (elements elem[l] * tl).  The reference
+// walks a row from its last element to its first and feeds ceil(result) of
+// element l+1 into element l as the upper bracket n1; results depend on the
+// previous element only through that bracket.  Here every lane searches its
+// elements with the brackets of the previous sweep until no bracket changes:
+// after k sweeps the last k elements are final, and a sweep that changes no
+// bracket reproduces the sequential walk exactly (typically 2-3 sweeps).
+__device__ void search_row(const double* list, int length, double tl, const double* elem, const double* init,
+                           double* out, double* out2, int len, int mode, double* lds) {
+    const int tid = threadIdx.x;
+    double El[kSearchEPT], ini[kSearchEPT], res[kSearchEPT];
+    size_t c[kSearchEPT];
+    const size_t L = (size_t)length;
+#pragma unroll
+    for (int k = 0; k < kSearchEPT; ++k) {
+        const int l = tid + k * kSearchThreads;
+        El[k] = 0.0; ini[k] = 1.0; res[k] = 0.0; c[k] = L;
+        if (l < len) {
+            El[k] = (mode == kSearchB) ? elem[l] * tl : tl * elem[l];
+            ini[k] = init[l];
+        }
+    }
+    if (mode == kSearchB) {
+#pragma unroll
+        for (int k = 0; k < kSearchEPT; ++k) {
+            const int l = tid + k * kSearchThreads;
+            if (l < len) res[k] = bsearch_elem(list, L, El[k]);
+        }
+    } else {
+        for (int sweep = 0; sweep <= len + 1; ++sweep) {
+#pragma unroll
+            for (int k = 0; k < kSearchEPT; ++k) {
+                const int l = tid + k * kSearchThreads;
+                if (l < len) {
+                    res[k] = isearch_elem(list, L, El[k], ini[k], c[k]);
+                    lds[l] = res[k];
+                }
+            }
+            __syncthreads();
+            int changed = 0;
+#pragma unroll
+            for (int k = 0; k < kSearchEPT; ++k) {
+                const int l = tid + k * kSearchThreads;
+                if (l < len) {
+                    size_t nc = L;
+                    if (l + 1 < len) {
+                        const double cv = ceil(lds[l + 1]);
+                        nc = (cv >= (double)L) ? L : (cv < 1.0 ? 1 : (size_t)cv);
+                    }
+                    changed |= (nc != c[k]);
+                    c[k] = nc;
+                }
+            }
+            if (!__syncthreads_or(changed)) break;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kSearchEPT; ++k) {
+        const int l = tid + k * kSearchThreads;
+        if (l < len) {
+            out[l] = res[k];
+            if (out2) out2[l] = res[k];
+        }
+    }
+}
+
+// Grid: N + 1 workgroups.  Blocks 0..N-1: phi2 row k (B2 positions); block N: the
+// theta row (B1 positions) followed by the B1 prologue.
+__global__ __launch_bounds__(kSearchThreads) void search_kernel(Tables T, History H, Work W, Model M, SearchArgs a) {
+    __shared__ double lds[kMaxN];
+    __shared__ double red[kSearchThreads / kWave];
+    const int N = T.N;
+    const int b = blockIdx.x;
+    if (b < N) {
+        double* out = W.posB2 + (size_t)b * N;
+        double* out2 = a.saveB2 ? a.saveB2 + (size_t)b * N : nullptr;
+        if (a.mode == kSearchSame) {
+            if (out2)
+                for (int l = threadIdx.x; l < N; l += kSearchThreads) out2[l] = out[l];
+            return;
+        }
+        if (a.mode == kSearchCopy) {
+            const double* src = a.srcB2 + (size_t)b * N;
+            for (int l = threadIdx.x; l < N; l += kSearchThreads) {
+                out[l] = src[l];
+                if (out2) out2[l] = src[l];
+            }
+            return;
+        }
+        search_row(H.t1, a.T1, a.tl, T.phi2 + (size_t)b * N, out, out, out2, N, a.mode, lds);
+        return;
+    }
+    // ---- theta row -----------------------------------------------------------------
+    double* pos = W.posB1;
+    if (a.mode == kSearchCopy) {
+        for (int l = threadIdx.x; l < N; l += kSearchThreads) pos[l] = a.srcB1[l];
+    } else if (a.mode != kSearchSame) {
+        search_row(H.t1, a.T1, a.tl, T.theta, pos, pos, nullptr, N, a.mode, lds);
+    }
+    __syncthreads();
+    if (a.saveB1)
+        for (int l = threadIdx.x; l < N; l += kSearchThreads) a.saveB1[l] = pos[l];
+    // maxCeil = ceil(max pos) - 1, at least 1 (interpolation_core.cpp:58-67)
+    double mx = -INFINITY;
+    for (int l = threadIdx.x; l < N; l += kSearchThreads) mx = fmax(mx, pos[l]);
+    mx = wave_max(mx);
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    mx = red[0];
+    for (int w = 1; w < kSearchThreads / kWave; ++w) mx = fmax(mx, red[w]);
+    size_t maxCeil = (size_t)ceil(mx) - 1;
+    if (maxCeil < 1) maxCeil = 1;
+    const size_t t1len = (size_t)a.T1;
+    const bool live = a.tl > 0;
+    for (int i = threadIdx.x; i < N; i += kSearchThreads) {
+        size_t f = (size_t)floor(pos[i]);
+        if (f < 1) f = 1;
+        else if (f > maxCeil) f = maxCeil;
+        if (f > t1len - 1) f = t1len > 1 ? t1len - 1 : 1;  // memory guard; never binds on valid positions
+        const double d = (double)f - pos[i];
+        W.Floor[i] = (int)f;
+        W.diff[i] = d;
+        // rInt: Hermite of rvec/drvec at the B1 positions (index_vec.cpp:138-156)
+        double ri;
+        if (!live) {
+            ri = H.r[0];
+        } else {
+            const double d2 = d * d, d3 = d * (d * d);
+            if (f < t1len - 1)
+                ri = (1 - 3 * d2 - 2 * d3) * H.r[f - 1] + (3 * d2 + 2 * d3) * H.r[f] - (d + 2 * d2 + d3) * H.dr[f] -
+                     (d2 + d3) * H.dr[f + 1] / H.rho[f + 1];
+            else
+                ri = (1 - d2) * H.r[f - 1] + d2 * H.r[f] - (d + d2) * H.dr[f];
+        }
+        W.rInt[i] = ri;
+    }
+}
+
+// ---- fused RHS rows ----------------------------------------------------------------
+constexpr int kRowThreads = 256;
+
+// Consecutive workgroups are dispatched round-robin over the 8 XCDs; give every
+// XCD a contiguous range of logical blocks (adjacent rows share history rows and
+// weight cache lines in that XCD's L2).
+__device__ inline int xcd_remap(int b, int nb) {
+    constexpr int X = 8;
+    if (nb % X) return b;
+    return (b % X) * (nb / X) + b / X;
+}
+
+template <bool LOGR>
+__device__ inline double2 stencil_last(const double* __restrict__ w, long long P, long long k, int m, int s,
+                                       const double2* __restrict__ vl) {
+    double a = 0.0, b = 0.0;
+    if (!LOGR) {
+        for (int d = 0; d < m; ++d) {
+            const double wd = w[(long long)d * P + k];
+            const double2 v = vl[s + d];
+            a = a + wd * v.x;
+            b = b + wd * v.y;
+        }
+        return make_double2(a, b);
+    }
+    double lin = 0.0, lsum = 0.0;
+    bool invalid = false;
+    for (int d = 0; d < m; ++d) {
+        const double wd = w[(long long)d * P + k];
+        const double2 v = vl[s + d];
+        a = a + wd * v.x;
+        lin += wd * v.y;
+        if (v.y > 0.0) lsum += wd * log(v.y);
+        else invalid = true;
+    }
+    return make_double2(a, invalid ? lin : exp(lsum));
+}
+
+template <bool LOGR>
+__global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History H, Work W, Model M, RhsArgs a) {
+    extern __shared__ double2 vlast[];  // last history row, N entries
+    __shared__ double red[kSums * (kRowThreads / kWave)];
+    const int N = T.N, m = T.m;
+    const long long P = T.P;
+    const int nb = gridDim.x;
+    const int lb = xcd_remap(blockIdx.x, nb);
+    const int j = lb / a.split;
+    const int chunk = lb % a.split;
+    const int cw = (N + a.split - 1) / a.split;
+    const int c0 = chunk * cw, c1 = min(N, c0 + cw);
+    const int T1 = a.T1;
+    const bool live = a.tl > 0;
+    const double2* Vl = H.V + (size_t)(T1 - 1) * N;
+    for (int l = threadIdx.x; l < N; l += kRowThreads) vlast[l] = Vl[l];
+    __syncthreads();
+
+    // B1 row data (row-uniform): Floor / diff of theta_j * t
+    const int F = W.Floor[j];
+    const double dB = W.diff[j];
+    const double in3 = dB * dB, in4 = in3 * dB;
+    const bool b1int = F < T1 - 1;
+    const double b1a = 1 - 3 * in3 - 2 * in4, b1b = 3 * in3 + 2 * in4, b1c = dB + 2 * in3 + in4, b1d = in3 + in4;
+    const double b1rho = b1int ? H.rho[F + 1] : 1.0;
+    const double2* VF0 = H.V + (size_t)(F - 1) * N;
+    const double2* VF1 = H.V + (size_t)F * N;
+    const double2* DF1 = H.D + (size_t)F * N;
+    const double2* DF2 = H.D + (size_t)(b1int ? F + 1 : F) * N;
+    // B2 time clamp from the last position of the whole phi2 block (index_mat.cpp:33)
+    size_t cap_hi = (size_t)(W.posB2[P - 1] - 0.5);
+    if (cap_hi > (size_t)(T1 - 1)) cap_hi = (size_t)(T1 - 1);  // memory guard
+    const double c00 = H.V[0].x, c01 = H.V[0].y;
+
+    double acc[kSums];
+#pragma unroll
+    for (int q = 0; q < kSums; ++q) acc[q] = 0.0;
+
+    for (int i = c0 + threadIdx.x; i < c1; i += kRowThreads) {
+        const long long k = (long long)j * N + i;
+        double qkA1, qrA1, qkA2, qrA2, qkB1, qrB1, qkB2, qrB2;
+        if (!live) {
+            qkA1 = qkA2 = qkB1 = qkB2 = c00;
+            qrA1 = qrA2 = qrB1 = qrB2 = c01;
+        } else {
+            // A1 / A2: stencils on the last row (index_vec.cpp:15-61)
+            const double2 a1 = stencil_last<LOGR>(T.w[0], P, k, m, T.inds[0][k], vlast);
+            const double2 a2 = stencil_last<LOGR>(T.w[1], P, k, m, T.inds[1][k], vlast);
+            qkA1 = a1.x; qrA1 = a1.y; qkA2 = a2.x; qrA2 = a2.y;
+            // B1: cubic Hermite in time along column i (index_vec.cpp:63-136)
+            const double2 y0 = VF0[i], y1 = VF1[i], dy1 = DF1[i];
+            if (b1int) {
+                const double2 dy2 = DF2[i];
+                qkB1 = b1a * y0.x + b1b * y1.x - b1c * dy1.x - b1d * dy2.x / b1rho;
+                if (!LOGR) {
+                    qrB1 = b1a * y0.y + b1b * y1.y - b1c * dy1.y - b1d * dy2.y / b1rho;
+                } else if (y0.y > 0.0 && y1.y > 0.0) {
+                    qrB1 = exp(b1a * log(y0.y) + b1b * log(y1.y) - b1c * (dy1.y / y0.y) - (b1d / b1rho) * (dy2.y / y1.y));
+                } else {
+                    qrB1 = b1a * y0.y + b1b * y1.y - b1c * dy1.y - b1d * dy2.y / b1rho;
+                }
+            } else {
+                qkB1 = (1 - in3) * y0.x - (dB + in3) * dy1.x + in3 * y1.x;
+                if (!LOGR || !(y0.y > 0.0 && y1.y > 0.0))
+                    qrB1 = (1 - in3) * y0.y - (dB + in3) * dy1.y + in3 * y1.y;
+                else
+                    qrB1 = exp((1 - in3) * log(y0.y) + in3 * log(y1.y) - (dB + in3) * (dy1.y / y0.y));
+            }
+            // B2: theta stencil on history rows ix-1, ix, ix+1 + Hermite (index_mat.cpp:18-91)
+            const double px = W.posB2[k];
+            size_t ix = (size_t)px;
+            if (cap_hi < ix) ix = cap_hi;
+            if (ix < 1) ix = 1;
+            const double u = px - ix, u2 = u * u, u3 = u2 * u;
+            const int s = T.inds[2][k];
+            const double2* R0 = H.V + (size_t)(ix - 1) * N + s;
+            const double2* R1 = H.V + (size_t)ix * N + s;
+            const double2* S1 = H.D + (size_t)ix * N + s;
+            const double* wb = T.w[2] + k;
+            const bool b2int = ix < (size_t)(T1 - 1);
+            double y0k = 0, y0r = 0, y1k = 0, y1r = 0, e1k = 0, e1r = 0, e2k = 0, e2r = 0;
+            if (b2int) {
+                const double2* S2 = H.D + (size_t)(ix + 1) * N + s;
+                for (int d = 0; d < m; ++d) {
+                    const double wd = wb[(long long)d * P];
+                    const double2 v0 = R0[d], v1 = R1[d], g1 = S1[d], g2 = S2[d];
+                    y0k = y0k + wd * v0.x; y0r = y0r + wd * v0.y;
+                    y1k = y1k + wd * v1.x; y1r = y1r + wd * v1.y;
+                    e1k = e1k + wd * g1.x; e1r = e1r + wd * g1.y;
+                    e2k = e2k + wd * g2.x; e2r = e2r + wd * g2.y;
+                }
+                const double rr = H.rho[ix + 1];
+                const double h0 = 1 - 3 * u2 + 2 * u3, h1 = u - 2 * u2 + u3, h2 = 3 * u2 - 2 * u3, h3 = -u2 + u3;
+                qkB2 = h0 * y0k + h1 * e1k + h2 * y1k + h3 * e2k / rr;
+                if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
+                    qrB2 = h0 * y0r + h1 * e1r + h2 * y1r + h3 * e2r / rr;
+                else
+                    qrB2 = exp(h0 * log(y0r) + h1 * (e1r / y0r) + h2 * log(y1r) + (h3 / rr) * (e2r / y1r));
+            } else {
+                for (int d = 0; d < m; ++d) {
+                    const double wd = wb[(long long)d * P];
+                    const double2 v0 = R0[d], v1 = R1[d], g1 = S1[d];
+                    y0k = y0k + wd * v0.x; y0r = y0r + wd * v0.y;
+                    y1k = y1k + wd * v1.x; y1r = y1r + wd * v1.y;
+                    e1k = e1k + wd * g1.x; e1r = e1r + wd * g1.y;
+                }
+                qkB2 = (1 - u2) * y0k + u2 * y1k + (u - u2) * e1k;
+                if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
+                    qrB2 = (1 - u2) * y0r + u2 * y1r + (u - u2) * e1r;
+                else
+                    qrB2 = exp((1 - u2) * log(y0r) + u2 * log(y1r) + (u - u2) * (e1r / y0r));
+            }
+        }
+        // self-energies (math_sigma.cpp:7-62)
+        const double sKA1 = fl1(M, qkA1), sRA1 = fl2(M, qkA1) * qrA1;
+        const double sRA2 = fl2(M, qkA2) * qrA2;
+        const double sKB1 = fl1(M, qkB1), sRB1 = fl2(M, qkB1) * qrB1;
+        const double sKB2 = fl1(M, qkB2), sRB2 = fl2(M, qkB2) * qrB2;
+        if (a.fields) {
+            double* f = a.fields;
+            const double v[15] = {qkA1, qrA1, qkA2, qrA2, qkB1, qrB1, qkB2, qrB2,
+                                  sKA1, sRA1, sRA2, sKB1, sRB1, sKB2, sRB2};
+#pragma unroll
+            for (int q = 0; q < 15; ++q) f[(long long)q * P + k] = v[q];
+        }
+        if (i == 0) W.qkb1c0[j] = qkB1;
+        // quadratures of time_steps.cpp:36-45, 97-98 (weights int.dat)
+        const double wi = T.integ[i];
+        acc[0] += wi * sRA2 * qkB2;  // ConvR(SigmaRA2, QKB2)
+        acc[1] += wi * sRA1 * qkB1;  // ConvA(SigmaRA1, QKB1)
+        acc[2] += wi * sKA1 * qrB1;  // ConvA(SigmaKA1, QRB1)
+        acc[3] += wi * qrA2 * sKB2;  // ConvR(QRA2, SigmaKB2)
+        acc[4] += wi * qrA1 * sKB1;  // ConvA(QRA1, SigmaKB1)
+        acc[5] += wi * qkA1 * sRB1;  // ConvA(QKA1, SigmaRB1)
+        acc[6] += wi * sRA2 * qrB2;  // ConvR(SigmaRA2, QRB2)
+        acc[7] += wi * qrA2 * sRB2;  // ConvR(QRA2, SigmaRB2)
+    }
+    block_sum<kSums, kRowThreads>(acc, red);
+    if (threadIdx.x < kSums) W.partial[((size_t)j * a.split + chunk) * kSums + threadIdx.x] = acc[threadIdx.x];
+}
+
+// ---- finish: RHS epilogue + RK stage -------------------------------------------------
+constexpr int kFinThreads = 512;
+constexpr int kFinEPT = kMaxN / kFinThreads;
+
+__global__ __launch_bounds__(kFinThreads) void finish_kernel(Tables T, History H, Work W, Model M, FinishArgs a) {
+    __shared__ double red[8 * (kFinThreads / kWave)];
+    const int N = T.N;
+    const int n = a.stage;
+    const double t = a.t_rhs;
+    const double2* last = H.V + (size_t)a.lr * N;
+    const double qK0 = last[0].x;
+    const double s1 = fl1(M, qK0) / M.T0, s2 = -W.rInt[N - 1], s3 = qK0 / M.T0, s4 = 2 * M.Gamma;
+    double hk[kFinEPT], hr[kFinEPT];
+#pragma unroll
+    for (int e = 0; e < kFinEPT; ++e) {
+        const int j = threadIdx.x + e * kFinThreads;
+        hk[e] = hr[e] = 0.0;
+        if (j >= N) continue;
+        double C[kSums];
+#pragma unroll
+        for (int q = 0; q < kSums; ++q) C[q] = 0.0;
+        for (int c = 0; c < a.split; ++c) {
+            const double* pp = W.partial + ((size_t)j * a.split + c) * kSums;
+#pragma unroll
+            for (int q = 0; q < kSums; ++q) C[q] += pp[q];
+        }
+        const double th = T.theta[j];
+        const double fA = t * th, fR = t * (1 - th);
+        const double qK = last[j].x, qR = last[j].y, ri = W.rInt[j], b1 = W.qkb1c0[j];
+        double d1 = b1 * s1 + qK * s2;
+        d1 = d1 + C[0] * fR;
+        d1 = d1 + C[1] * fA;
+        d1 = d1 + C[2] * fA;
+        double d2 = fl1(M, b1) * s3 + qR * s4;
+        d2 = d2 + C[3] * fR;
+        d2 = d2 + C[4] * fA;
+        d2 = d2 + C[5] * fA;
+        d2 = d2 - qK * ri;
+        hk[e] = d1 + d2 * th;
+        hr[e] = (qR * s2 + C[6] * fR) + (qR * ri - C[7] * fR) * th;
+        W.hK[(size_t)n * N + j] = hk[e];
+        W.hR[(size_t)n * N + j] = hr[e];
+    }
+    if (a.mode == 0) return;
+
+    // drstep2 at stage 0 (time_steps.cpp:117-125) on the step's start row
+    if (n == 0) {
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        double dsK0 = 0.0, sK0 = 0.0;
+#pragma unroll
+        for (int e = 0; e < kFinEPT; ++e) {
+            const int j = threadIdx.x + e * kFinThreads;
+            if (j >= N) continue;
+            const double qK = last[j].x, qR = last[j].y, w = T.integ[j];
+            const double sK = fl1(M, qK), sR = fl2(M, qK) * qR;
+            const double dsK = fl2(M, qK) * hk[e] + 0.0 * hr[e];
+            const double dsR = (fl3(M, qK) * qR) * hk[e] + fl2(M, qK) * hr[e];
+            v[0] += (1.0 * w) * sR * qK;
+            v[1] += (1.0 * w) * sK * qR;
+            v[2] += (t * w) * dsR * qK;
+            v[3] += (t * w) * dsK * qR;
+            v[4] += (t * w) * sR * hk[e];
+            v[5] += (t * w) * sK * hr[e];
+            if (j == 0) { dsK0 = dsK; sK0 = sK; }
+        }
+        block_sum<6, kFinThreads>(v, red);
+        if (threadIdx.x == 0)
+            W.scal[0] = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + (dsK0 * qK0 + sK0 * hk[0]) / M.T0;
+        __syncthreads();
+    }
+    const double dr = W.scal[0];
+    // stage combination g_{n+1} = g_0 + sum_j h_j * (dt a_{n+1,j}) and the row write
+    const double2* g0 = H.V + (size_t)a.g0row * N;
+    double2* wv = H.V + (size_t)a.wrow * N;
+    double2* wd = H.D + (size_t)a.wrow * N;
+    double rr[2] = {0.0, 0.0};
+    double err[1] = {0.0};
+    double gK0 = 0.0, sKg0 = 0.0;
+#pragma unroll
+    for (int e = 0; e < kFinEPT; ++e) {
+        const int j = threadIdx.x + e * kFinThreads;
+        if (j >= N) continue;
+        const double2 g = g0[j];
+        double gK = g.x, gR = g.y;
+        for (int q = 0; q < a.nterms; ++q) {
+            const double hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
+            const double hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
+            gK = gK + hq * a.coef[q];
+            gR = gR + hqr * a.coef[q];
+        }
+        const double h0k = (n == 0) ? hk[e] : W.hK[j];
+        const double h0r = (n == 0) ? hr[e] : W.hR[j];
+        wv[j] = make_double2(gK, gR);
+        wd[j] = make_double2(a.tdiff * h0k, a.tdiff * h0r);
+        // rstep on the new row (time_steps.cpp:83-92)
+        const double w = T.integ[j];
+        const double sK = fl1(M, gK), sR = fl2(M, gK) * gR;
+        rr[0] += a.t_new * w * sR * gK;
+        rr[1] += a.t_new * w * sK * gR;
+        if (j == 0) { gK0 = gK; sKg0 = sK; }
+        if (a.last) {
+            double eK = g.x, eR = g.y;
+            for (int q = 0; q <= n; ++q) {
+                const double hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
+                const double hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
+                eK = eK + hq * a.coefE[q];
+                eR = eR + hqr * a.coefE[q];
+            }
+            err[0] += fabs(gK - eK) + fabs(gR - eR);
+        }
+    }
+    block_sum<2, kFinThreads>(rr, red);
+    if (a.last) block_sum<1, kFinThreads>(err, red);
+    if (threadIdx.x == 0) {
+        H.t1[a.wrow] = a.t_new;
+        H.rho[a.wrow] = a.rho_new;
+        H.r[a.wrow] = M.Gamma + rr[0] + rr[1] + sKg0 * gK0 / M.T0;
+        H.dr[a.wrow] = a.tdiff * dr;
+        if (a.last) W.scal[1] = err[0] + a.gt_err;
+    }
+}
+
+// rstep (what 0) / energy (what 1) on one history row
+__global__ __launch_bounds__(kFinThreads) void row_scalar_kernel(Tables T, History H, Work W, Model M, int row,
+                                                                 double t, int what) {
+    __shared__ double red[2 * (kFinThreads / kWave)];
+    const int N = T.N;
+    const double2* v = H.V + (size_t)row * N;
+    double s[2] = {0.0, 0.0};
+    for (int j = threadIdx.x; j < N; j += kFinThreads) {
+        const double qK = v[j].x, qR = v[j].y, w = T.integ[j];
+        const double sK = fl1(M, qK);
+        if (what == 0) {
+            s[0] += t * w * (fl2(M, qK) * qR) * qK;
+            s[1] += t * w * sK * qR;
+        } else {
+            s[0] += t * w * sK * qR;
+        }
+    }
+    block_sum<2, kFinThreads>(s, red);
+    if (threadIdx.x == 0) {
+        const double qK0 = v[0].x;
+        if (what == 0) W.scal[2] = M.Gamma + s[0] + s[1] + fl1(M, qK0) * qK0 / M.T0;
+        else W.scal[2] = -(s[0] + fl1(M, qK0) / M.T0);
+    }
+}
+
+// sparsifyNscale values (sparsify_utils.cpp:19-49): one workgroup per interior row i
+__global__ __launch_bounds__(kRowThreads) void sparsify_values_kernel(Tables T, History H, int T1, double* val) {
+    __shared__ double red[kRowThreads / kWave];
+    const int i = blockIdx.x + 2;
+    if (i + 1 >= T1) return;
+    const int N = T.N;
+    const double tleft = H.t1[i - 2], tmid = H.t1[i];
+    const double td1 = H.t1[i - 1] - tleft, td2 = tmid - tleft, td3 = H.t1[i + 1] - tmid;
+    double s[1] = {0.0};
+    for (int j = threadIdx.x; j < N; j += kRowThreads) {
+        const double2 a = H.V[(size_t)i * N + j], b = H.V[(size_t)(i - 2) * N + j];
+        const double2 d1 = H.D[(size_t)(i - 1) * N + j], d2 = H.D[(size_t)(i + 1) * N + j];
+        s[0] += fabs(td2 / 12.0 * (2 * (a.x - b.x) - td2 * (d1.x / td1 + d2.x / td3)));
+        s[0] += fabs(td2 / 12.0 * (2 * (a.y - b.y) - td2 * (d1.y / td1 + d2.y / td3)));
+    }
+    block_sum<1, kRowThreads>(s, red);
+    if (threadIdx.x == 0) {
+        const double Rt = H.r[i] - H.r[i - 2];
+        val[i] = s[0] + fabs(td2 / 12.0 * (2 * Rt - td2 * (H.dr[i - 1] / td1 + H.dr[i + 1] / td3)));
+    }
+}
+
+__global__ __launch_bounds__(kRowThreads) void gather_rows_kernel(Tables T, History src, History dst, const int* keep,
+                                                                  int nkeep) {
+    const int r = blockIdx.x;
+    if (r >= nkeep) return;
+    const int s = keep[r];
+    const int N = T.N;
+    for (int j = threadIdx.x; j < N; j += kRowThreads) {
+        dst.V[(size_t)r * N + j] = src.V[(size_t)s * N + j];
+        dst.D[(size_t)r * N + j] = src.D[(size_t)s * N + j];
+    }
+    if (threadIdx.x == 0) {
+        dst.t1[r] = src.t1[s];
+        dst.rho[r] = src.rho[s];
+        dst.r[r] = src.r[s];
+        dst.dr[r] = src.dr[s];
+    }
+}
+
+}  // namespace
+
+void launch_search(const Tables& T, const History& H, const Work& W, const Model& M, const SearchArgs& a,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(search_kernel, dim3(T.N + 1), dim3(kSearchThreads), 0, s, T, H, W, M, a);
+}
+
+void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
+                     hipStream_t s) {
+    const dim3 grid((unsigned)(T.N * a.split));
+    const size_t lds = (size_t)T.N * sizeof(double2);
+    if (a.log_response)
+        hipLaunchKernelGGL(rhs_rows_kernel<true>, grid, dim3(kRowThreads), lds, s, T, H, W, M, a);
+    else
+        hipLaunchKernelGGL(rhs_rows_kernel<false>, grid, dim3(kRowThreads), lds, s, T, H, W, M, a);
+}
+
+void launch_finish(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kFinThreads), 0, s, T, H, W, M, a);
+}
+
+void launch_row_scalar(const Tables& T, const History& H, const Work& W, const Model& M, int row, double t,
+                       int what, hipStream_t s) {
+    hipLaunchKernelGGL(row_scalar_kernel, dim3(1), dim3(kFinThreads), 0, s, T, H, W, M, row, t, what);
+}
+
+void launch_sparsify_values(const Tables& T, const History& H, int T1, double* val, hipStream_t s) {
+    if (T1 < 4) return;
+    hipLaunchKernelGGL(sparsify_values_kernel, dim3(T1 - 3), dim3(kRowThreads), 0, s, T, H, T1, val);
+}
+
+void launch_gather_rows(const Tables& T, const History& src, const History& dst, const int* keep, int nkeep,
+                        hipStream_t s) {
+    if (nkeep <= 0) return;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(nkeep), dim3(kRowThreads), 0, s, T, src, dst, keep, nkeep);
+}
+
+}  // namespace dmfe

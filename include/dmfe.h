@@ -1,0 +1,216 @@
+/* dmfe.h - C ABI of the MI355X (gfx950) implementation of the per-RK-step
+ * renormalized DMFT update for the spherical mixed p-spin model.
+ *
+ * The reference has no plugin/FFI layer: its hot path is a set of C++ free
+ * functions on global state (SimulationConfig config, SimulationData* sim,
+ * RKData* rk; main.cpp:32-37).  Each entry point below names the reference
+ * function or call site it replaces.  Conventions:
+ *   - plain pointers and sizes only; every function returns a DMFE_* status
+ *     (0 on success) and dmfe_last_error() describes the last failure;
+ *   - host buffers are caller-owned and only borrowed during the call;
+ *   - a dmfe_ctx owns its device memory and one HIP stream; a context is not
+ *     thread-safe, distinct contexts are independent;
+ *   - all arithmetic is FP64; stencil/time searches are bit-exact with the
+ *     reference CPU path (search_utils.cpp:13-131).
+ * Library: dynamite_amd/libdmfe.so (built by dynamite_amd/build.py).
+ */
+#ifndef DMFE_H
+#define DMFE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DMFE_OK 0
+#define DMFE_ERR_ARG (-1)     /* invalid argument (reference: std::invalid_argument) */
+#define DMFE_ERR_HIP (-2)     /* HIP runtime error or no device */
+#define DMFE_ERR_STATE (-3)   /* call not valid in the current state */
+#define DMFE_ERR_IO (-4)      /* file read/write failure */
+#define DMFE_ERR_NOMEM (-5)   /* device allocation failed */
+
+/* interpolation methods of a grid package (grid_cli.cpp:108-161) */
+#define DMFE_INTERP_POLY 0
+#define DMFE_INTERP_RATIONAL 1
+#define DMFE_INTERP_BSPLINE 2
+
+/* integrators (rk_data.hpp:12, runge_kutta.cpp:254-261): rk->init values */
+#define DMFE_RK54 1
+#define DMFE_SSPRK104 2
+
+/* interpolation position modes (interpolation_core.cpp:22-29) */
+#define DMFE_POS_BSEARCH 0  /* interpolate() with no positions: binary search of theta*t, phi2*t */
+#define DMFE_POS_ISEARCH 1  /* interpolate(posOld, posOld, false): seeded interpolation search */
+#define DMFE_POS_SAME 2     /* interpolate(posOld, posOld, true): reuse positions */
+
+/* Last error message of the calling thread ("" if none). */
+const char* dmfe_last_error(void);
+
+/* Library version string. */
+const char* dmfe_version(void);
+
+/* ---------------------------------------------------------------- grid packages */
+
+/* Generation parameters: the `grid` subcommand options (grid_cli.cpp:18-74). */
+typedef struct dmfe_grid_spec {
+    int64_t N;          /* -L, grid length */
+    double Tmax;        /* -M */
+    double alpha;       /* -a */
+    double delta;       /* -D */
+    int32_t spline_order; /* -s, int.dat quadrature degree */
+    int32_t method;     /* -m, DMFE_INTERP_* */
+    int32_t order;      /* -o, interpolation order n */
+    int32_t fh_stencil; /* -f, Floater-Hormann window (rational), <=0: n+1 */
+} dmfe_grid_spec;
+
+/* Entries per target of the stencil tables: poly n+1, rational the FH window,
+ * bspline N (index_vec.cpp:19 infers it as |weights|/|inds|). */
+int64_t dmfe_grid_stencil_width(int64_t N, int method, int order, int fh_stencil);
+
+/* Generates a grid package in memory.  Replaces maybe_handle_grid_cli()
+ * (grid_cli.cpp:35-233).  Caller allocates: theta, integ [N]; phi1, phi2,
+ * posA1, posA2, posB2 [N*N] (phi1 and the pos arrays may be NULL);
+ * inds [3*N*N] (A1, A2, B2 blocks; -1 for bspline); weights [3*N*N*m]
+ * (entry-major, m = dmfe_grid_stencil_width). */
+int dmfe_grid_generate(const dmfe_grid_spec* spec, double* theta, double* integ, double* phi1, double* phi2,
+                       double* posA1, double* posA2, double* posB2, int64_t* inds, double* weights);
+
+/* theta grid in extended precision (long double, the generator's internal
+ * precision; theta_grid.cpp:71-125). */
+int dmfe_grid_theta_ld(int64_t N, double Tmax, double alpha, double delta, long double* theta);
+
+/* Writes Grid_data/<L>-format text files into dir (grid_io.cpp:26-328).
+ * Optional arrays may be NULL; params_text (grid_params.txt body) may be NULL. */
+int dmfe_grid_write_dir(const char* dir, int64_t N, int64_t m, const double* theta, const double* integ,
+                        const double* phi1, const double* phi2, const double* posA1, const double* posA2,
+                        const double* posB2, const int64_t* inds, const double* weights, const char* params_text);
+
+/* Sizes of a grid package on disk: N from theta.dat, m = |weightsA1y|/|indsA1y|. */
+int dmfe_grid_read_info(const char* dir, int64_t* N, int64_t* m);
+
+/* Reads a grid package (import(), io_input.cpp:90-110).  phi2 may be NULL; if
+ * phi2.dat is absent it is rebuilt from theta (phi_grid.cpp:17-34). */
+int dmfe_grid_read_dir(const char* dir, int64_t N, int64_t m, double* theta, double* integ, double* phi2,
+                       int64_t* inds, double* weights);
+
+/* ------------------------------------------------------------- step context */
+
+/* Grid tables handed to a context (borrowed during dmfe_ctx_create only).
+ * Layout = the reference's SimulationData host vectors (simulation_data.hpp:93-94). */
+typedef struct dmfe_grid {
+    int64_t N;                 /* config.len */
+    int64_t m;                 /* stencil width */
+    const double* theta;       /* h_theta [N] */
+    const double* integ;       /* h_integ [N] (int.dat) */
+    const double* phi2;        /* h_phi2 [N*N] */
+    const int64_t* inds[3];    /* h_indsA1y, h_indsA2y, h_indsB2y [N*N]; <0 = dense row (bspline) */
+    const double* weights[3];  /* h_weightsA1y, h_weightsA2y, h_weightsB2y [N*N*m] entry-major */
+} dmfe_grid;
+
+/* Model and control parameters (SimulationConfig, config.hpp:18-56). */
+typedef struct dmfe_params {
+    int32_t p;            /* -p */
+    int32_t p2;           /* -q */
+    double lambda;        /* -l */
+    double T0;            /* -T, initial temperature (inf -> 1e50) */
+    double Gamma;         /* -G, bath temperature */
+    double delta_t_min;   /* -d */
+    double delta_max;     /* -e */
+    double rmax[11];      /* stability radii per method (config.hpp:33) */
+    int32_t log_response; /* -R, interpolate QR in log space */
+    int32_t reserved;
+} dmfe_params;
+
+/* Fills *p with the reference defaults (config.hpp:18-56). */
+void dmfe_params_default(dmfe_params* p);
+
+typedef struct dmfe_ctx dmfe_ctx;
+
+/* Creates a context on the current HIP device: uploads the grid tables and
+ * reserves a history arena of `rows_hint` rows (grown on demand).
+ * Replaces init()'s table upload (initialization.cpp:177-186, copyVectorsToGPU). */
+int dmfe_ctx_create(const dmfe_grid* grid, const dmfe_params* params, int64_t rows_hint, dmfe_ctx** out);
+int dmfe_ctx_destroy(dmfe_ctx* ctx);
+
+/* Quench initial condition and first interpolation: the "New simulation" branch
+ * of init() (initialization.cpp:204-223) + RK choice and interpolate() (:277-283). */
+int dmfe_init_quench(dmfe_ctx* ctx);
+
+/* History access, byte-compatible with the data.bin v1 arrays
+ * (io_save_binary.cpp:66-100): t1grid[T1], QK/QR/dQK/dQR [T1*N] row-major, rvec,
+ * drvec [T1].  dtratio may be NULL on set: then it is rebuilt as in
+ * loadSimulationStateBinary (io_input.cpp:169-172). */
+int dmfe_get_T1(const dmfe_ctx* ctx, int64_t* T1);
+int dmfe_set_state(dmfe_ctx* ctx, int64_t T1, const double* t1grid, const double* QK, const double* QR,
+                   const double* dQK, const double* dQR, const double* rvec, const double* drvec,
+                   const double* dtratio);
+int dmfe_get_state(dmfe_ctx* ctx, double* t1grid, double* QK, double* QR, double* dQK, double* dQR,
+                   double* rvec, double* drvec, double* dtratio);
+
+/* Control scalars (config.delta_t, delta, delta_old, loop, rk->init). */
+int dmfe_set_control(dmfe_ctx* ctx, double delta_t, double delta, double delta_old, int64_t loop, int method);
+int dmfe_get_control(const dmfe_ctx* ctx, double* delta_t, double* delta, double* delta_old, int64_t* loop,
+                     int* method);
+
+/* interpolate() / interpolate(posOld, posOld, same) (interpolation_core.cpp:19-114):
+ * computes the time positions of theta*t and phi2*t (posB1xOld, posB2xOld) and
+ * the B1 prologue (Floor, diff, rInt).  The N*N interpolated fields are not
+ * materialised: they are rebuilt inside the RHS kernel. */
+int dmfe_interpolate(dmfe_ctx* ctx, int mode);
+
+/* Positions left by the last interpolation: posB1 [N], posB2 [N*N]. */
+int dmfe_get_positions(dmfe_ctx* ctx, double* posB1, double* posB2);
+int dmfe_set_positions(dmfe_ctx* ctx, const double* posB1, const double* posB2);
+
+/* Materialises the interpolated fields of the last interpolation (test/debug):
+ * out [15*N*N + N] = QKA1 QRA1 QKA2 QRA2 QKB1 QRB1 QKB2 QRB2 SigmaKA1 SigmaRA1
+ * SigmaRA2 SigmaKB1 SigmaRB1 SigmaKB2 SigmaRB2, then rInt (the sim->h_* fields
+ * written by interpolate(), interpolation_core.cpp:35-113). */
+int dmfe_get_fields(dmfe_ctx* ctx, double* out);
+
+/* QKstep() and QRstep() (time_steps.cpp:27-47, 94-100) on the current
+ * interpolation state: hK, hR [N]. */
+int dmfe_rhs(dmfe_ctx* ctx, double* hK, double* hR);
+
+/* rstep() (time_steps.cpp:83-92) on the last history row. */
+int dmfe_rstep(dmfe_ctx* ctx, double* r);
+
+/* update() (runge_kutta.cpp:254-261): one RK54 or SSPRK104 step; *err = the
+ * embedded error norm (runge_kutta.cpp:241-251). */
+int dmfe_rk_step(dmfe_ctx* ctx, double* err);
+
+/* One iteration of the CPU main loop (simulation_runner.cpp:125-304 without
+ * I/O): proposed t, update(), loop++, every 1e5 loops sparsify + interpolate +
+ * method switch, then the adaptive dt rule.  *t_out = proposed time written to
+ * correlation.txt (:130, :331). */
+int dmfe_loop_iteration(dmfe_ctx* ctx, double* t_out);
+
+/* Runs loop iterations while t < tmax, loop < max_loop and delta_t >= delta_t_min
+ * (simulation_runner.cpp:125).  If corr is non-NULL it receives up to corr_cap
+ * rows (t, E, QK(t,0)) - the correlation.txt columns (:327-341).  *steps = number
+ * of iterations run. */
+int dmfe_run(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps);
+
+/* Energy at time t and QK(t,0) on the last history row (simulation_runner.cpp:308-337). */
+int dmfe_observables(dmfe_ctx* ctx, double t, double* energy, double* qk0);
+
+/* sparsifyNscale(threshold) (sparsify_utils.cpp:14-97, CPU-only build semantics). */
+int dmfe_sparsify(dmfe_ctx* ctx, double threshold);
+
+/* Kernel timing with HIP events recorded around every launch on the context's
+ * stream (no counterpart in the reference; used by bench.py).  enable != 0
+ * starts a fresh measurement.  kind: 0 search_kernel, 1 rhs_rows_kernel,
+ * 2 finish_kernel.  *total_ms = summed event time, *launches = launch count. */
+int dmfe_profile(dmfe_ctx* ctx, int enable);
+int dmfe_profile_read(dmfe_ctx* ctx, int kind, double* total_ms, int64_t* launches);
+
+/* Waits for all work queued on the context's stream. */
+int dmfe_synchronize(dmfe_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMFE_H */

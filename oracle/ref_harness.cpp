@@ -295,3 +295,10 @@ void ref_grid_weights(int which, int method, int n, int mfh, int64_t* inds, doub
 }
 
 }  // extern "C"
+
+// Replace the generator's theta (extended precision) so the weight / quadrature
+// code of the reference can be compared on exactly the same nodes.
+extern "C" void ref_grid_set_theta_ld(int64_t N, const long double* th) {
+    g_theta.assign(th, th + N);
+    generate_phi_grids(g_theta, g_phi1, g_phi2);
+}

@@ -1,0 +1,324 @@
+"""ctypes bindings of the test oracles (test infrastructure only).
+
+  Oracle      oracle/liboracle.so                (C restatement of the reference CPU path)
+  RefLib      oracle/_ref/libdmfe_ref_strict.so  (the reference's own sources + ref_harness.cpp)
+
+Plus the seeded synthetic history used by the parity tests and the benchmark.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libdmfe_ref_strict.so")
+REF_FAST_PATH = os.path.join(ROOT, "oracle", "_ref", "libdmfe_ref_fast.so")
+
+_dp = C.c_void_p
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def build_oracle():
+    """Builds oracle/liboracle.so (gcc) if missing."""
+    if not os.path.exists(ORACLE_PATH):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True, capture_output=True)
+    return ORACLE_PATH
+
+
+_SETUP_ARGT = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int64, C.c_int, C.c_int, C.c_double,
+               C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+
+
+def _setup_args(grid, p, p2, lam, T0, Gamma, dt_min, delta_max, log_resp):
+    return [grid.N, _p(grid.theta), _p(grid.integ), _p(grid.phi2), _p(grid.inds[0]), _p(grid.inds[1]),
+            _p(grid.inds[2]), _p(grid.weights[0]), _p(grid.weights[1]), _p(grid.weights[2]), grid.m, p, p2, lam,
+            T0, Gamma, dt_min, delta_max, log_resp]
+
+
+class _Common:
+    """Shared methods of the oracle and the reference harness (same call surface)."""
+
+    def _hist(self, T1):
+        N = self.N
+        a = [np.zeros(T1) for _ in range(4)] + [np.zeros(T1 * N) for _ in range(4)]
+        return a
+
+    def state(self):
+        T1 = self.T1
+        t1, rho, r, dr, QK, QR, dQK, dQR = self._hist(T1)
+        self._get_history(t1, rho, r, dr, QK, QR, dQK, dQR)
+        return dict(t1grid=t1, dtratio=rho, rvec=r, drvec=dr, QK=QK, QR=QR, dQK=dQK, dQR=dQR)
+
+    def fields(self):
+        P = self.N * self.N
+        out = np.zeros(15 * P + self.N)
+        self._get_fields(out)
+        names = ("QKA1", "QRA1", "QKA2", "QRA2", "QKB1", "QRB1", "QKB2", "QRB2",
+                 "SigmaKA1", "SigmaRA1", "SigmaRA2", "SigmaKB1", "SigmaRB1", "SigmaKB2", "SigmaRB2")
+        d = {n: out[i * P:(i + 1) * P] for i, n in enumerate(names)}
+        d["rInt"] = out[15 * P:]
+        return d
+
+
+class Oracle(_Common):
+    """C restatement (oracle/dmfe_oracle.c)."""
+
+    def __init__(self, grid, p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.8, dt_min=1e-5, delta_max=1e-10, log_resp=0):
+        L = C.CDLL(build_oracle())
+        self.L = L
+        L.or_create.argtypes = _SETUP_ARGT
+        L.or_create.restype = C.c_void_p
+        for f in ("or_rstep", "or_update", "or_loop_iteration", "or_energy", "or_drstep2", "or_flambda"):
+            getattr(L, f).restype = C.c_double
+        L.or_T1.restype = C.c_int64
+        L.or_T1.argtypes = [C.c_void_p]
+        L.or_energy.argtypes = [C.c_void_p, C.c_double]
+        L.or_flambda.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.or_drstep2.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double]
+        L.or_set_control.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.or_sparsify.argtypes = [C.c_void_p, C.c_double]
+        L.or_interpolate.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_history.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 8
+        L.or_bsearch.argtypes = [C.c_int64, _dp, C.c_int64, _dp, _dp]
+        L.or_isearch.argtypes = [C.c_int64, _dp, C.c_int64, C.c_int64, _dp, _dp, _dp]
+        for f in ("or_destroy", "or_init_quench", "or_update", "or_loop_iteration", "or_rstep", "or_rhs",
+                  "or_get_fields", "or_get_history", "or_get_control", "or_set_pos_old", "or_get_pos_old"):
+            if getattr(L, f).argtypes is None:
+                getattr(L, f).argtypes = None
+        self.N = grid.N
+        self.h = C.c_void_p(L.or_create(*_setup_args(grid, p, p2, lam, T0, Gamma, dt_min, delta_max, log_resp)))
+
+    def close(self):
+        if self.h:
+            self.L.or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def T1(self):
+        return self.L.or_T1(self.h)
+
+    def init_quench(self):
+        self.L.or_init_quench(self.h)
+
+    def set_history(self, t1grid, dtratio, rvec, drvec, QK, QR, dQK, dQR):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (t1grid, dtratio, rvec, drvec, QK, QR, dQK, dQR)]
+        self.L.or_set_history(self.h, len(arrs[0]), *[_p(a) for a in arrs])
+
+    def _get_history(self, *a):
+        self.L.or_get_history(self.h, *[_p(x) for x in a])
+
+    def _get_fields(self, out):
+        self.L.or_get_fields(self.h, _p(out))
+
+    def set_control(self, delta_t, delta, delta_old, loop, method):
+        self.L.or_set_control(self.h, delta_t, delta, delta_old, loop, method)
+
+    def control(self):
+        dt, de, do, lp, me = C.c_double(), C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        self.L.or_get_control(self.h, C.byref(dt), C.byref(de), C.byref(do), C.byref(lp), C.byref(me))
+        return dict(delta_t=dt.value, delta=de.value, delta_old=do.value, loop=lp.value, method=me.value)
+
+    def set_pos_old(self, b1, b2):
+        self.L.or_set_pos_old(self.h, _p(np.ascontiguousarray(b1)), _p(np.ascontiguousarray(b2)))
+
+    def positions(self):
+        b1, b2 = np.zeros(self.N), np.zeros(self.N * self.N)
+        self.L.or_get_pos_old(self.h, _p(b1), _p(b2))
+        return b1, b2
+
+    def interpolate(self, mode=0):
+        self.L.or_interpolate(self.h, mode)
+
+    def rhs(self):
+        hK, hR = np.zeros(self.N), np.zeros(self.N)
+        self.L.or_rhs(self.h, _p(hK), _p(hR))
+        return hK, hR
+
+    def rstep(self):
+        return self.L.or_rstep(self.h)
+
+    def drstep2(self, qK, qR, dqK, dqR, t):
+        return self.L.or_drstep2(self.h, _p(qK), _p(qR), _p(dqK), _p(dqR), t)
+
+    def update(self):
+        return self.L.or_update(self.h)
+
+    def loop_iteration(self):
+        return self.L.or_loop_iteration(self.h)
+
+    def energy(self, t):
+        return self.L.or_energy(self.h, t)
+
+    def sparsify(self, thr):
+        self.L.or_sparsify(self.h, thr)
+
+    def flambda(self, q, d):
+        return self.L.or_flambda(self.h, q, d)
+
+    def bsearch(self, lst, el):
+        lst = np.ascontiguousarray(lst, np.float64)
+        el = np.ascontiguousarray(el, np.float64)
+        out = np.zeros(len(el))
+        self.L.or_bsearch(len(lst), _p(lst), len(el), _p(el), _p(out))
+        return out
+
+    def isearch(self, lst, el, inits, rowlen):
+        lst = np.ascontiguousarray(lst, np.float64)
+        el = np.ascontiguousarray(el, np.float64)
+        inits = np.ascontiguousarray(inits, np.float64)
+        out = np.zeros(len(el))
+        self.L.or_isearch(len(lst), _p(lst), len(el), rowlen, _p(el), _p(inits), _p(out))
+        return out
+
+
+class RefLib(_Common):
+    """The reference's own CPU sources, built by oracle/Makefile (process-global state)."""
+
+    def __init__(self, grid, p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.8, dt_min=1e-5, delta_max=1e-10, log_resp=0,
+                 path=REF_PATH):
+        os.environ.setdefault("OMP_NUM_THREADS", "1")
+        L = C.CDLL(path)
+        self.L = L
+        L.ref_setup.argtypes = _SETUP_ARGT
+        for f in ("ref_rstep", "ref_update", "ref_loop_iteration", "ref_energy", "ref_drstep2"):
+            getattr(L, f).restype = C.c_double
+        L.ref_T1.restype = C.c_int64
+        L.ref_energy.argtypes = [C.c_double]
+        L.ref_drstep2.argtypes = [_dp, _dp, _dp, _dp, C.c_double]
+        L.ref_set_control.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.ref_sparsify.argtypes = [C.c_double]
+        L.ref_set_history.argtypes = [C.c_int64] + [_dp] * 8
+        L.ref_bsearch.argtypes = [C.c_int64, _dp, C.c_int64, _dp, _dp]
+        L.ref_isearch.argtypes = [C.c_int64, _dp, C.c_int64, _dp, _dp, _dp]
+        self.N = grid.N
+        L.ref_setup(*_setup_args(grid, p, p2, lam, T0, Gamma, dt_min, delta_max, log_resp))
+
+    @property
+    def T1(self):
+        return self.L.ref_T1()
+
+    def init_quench(self):
+        self.L.ref_init_quench()
+
+    def set_history(self, t1grid, dtratio, rvec, drvec, QK, QR, dQK, dQR):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (t1grid, dtratio, rvec, drvec, QK, QR, dQK, dQR)]
+        self.L.ref_set_history(len(arrs[0]), *[_p(a) for a in arrs])
+
+    def _get_history(self, *a):
+        self.L.ref_get_history(*[_p(x) for x in a])
+
+    def _get_fields(self, out):
+        self.L.ref_get_fields(_p(out))
+
+    def set_control(self, delta_t, delta, delta_old, loop, method):
+        self.L.ref_set_control(delta_t, delta, delta_old, loop, method)
+
+    def control(self):
+        dt, de, do, lp, me = C.c_double(), C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        self.L.ref_get_control(C.byref(dt), C.byref(de), C.byref(do), C.byref(lp), C.byref(me))
+        return dict(delta_t=dt.value, delta=de.value, delta_old=do.value, loop=lp.value, method=me.value)
+
+    def set_pos_old(self, b1, b2):
+        self.L.ref_set_pos_old(_p(np.ascontiguousarray(b1)), _p(np.ascontiguousarray(b2)))
+
+    def positions(self):
+        b1, b2 = np.zeros(self.N), np.zeros(self.N * self.N)
+        self.L.ref_get_pos_old(_p(b1), _p(b2))
+        return b1, b2
+
+    def interpolate(self, mode=0):
+        self.L.ref_interpolate(mode)
+
+    def rhs(self):
+        hK, hR = np.zeros(self.N), np.zeros(self.N)
+        self.L.ref_rhs(_p(hK), _p(hR))
+        return hK, hR
+
+    def rstep(self):
+        return self.L.ref_rstep()
+
+    def drstep2(self, qK, qR, dqK, dqR, t):
+        return self.L.ref_drstep2(_p(qK), _p(qR), _p(dqK), _p(dqR), t)
+
+    def update(self):
+        return self.L.ref_update()
+
+    def loop_iteration(self):
+        return self.L.ref_loop_iteration()
+
+    def energy(self, t):
+        return self.L.ref_energy(t)
+
+    def sparsify(self, thr):
+        self.L.ref_sparsify(thr)
+
+    def bsearch(self, lst, el):
+        lst = np.ascontiguousarray(lst, np.float64)
+        el = np.ascontiguousarray(el, np.float64)
+        out = np.zeros(len(el))
+        self.L.ref_bsearch(len(lst), _p(lst), len(el), _p(el), _p(out))
+        return out
+
+    def isearch(self, lst, el, inits, rowlen):
+        assert rowlen == self.N
+        lst = np.ascontiguousarray(lst, np.float64)
+        el = np.ascontiguousarray(el, np.float64)
+        inits = np.ascontiguousarray(inits, np.float64)
+        out = np.zeros(len(el))
+        self.L.ref_isearch(len(lst), _p(lst), len(el), _p(el), _p(inits), _p(out))
+        return out
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+SEED = 20261203
+
+
+def synthetic_history(theta, T1=None, t_scale=1e3, seed=SEED, noise=1e-3):
+    """Seeded synthetic history of the kernel microbenchmark.
+
+    Rows k = 0..T1-1 at t_k = t_scale * theta_k (theta resampled to T1 points when
+    T1 != N); QK_kj = exp(-|theta_k - theta_j| u_k) (1 + noise xi_kj), u_k ~ U[0.5, 2],
+    xi ~ U[-1, 1]; QR likewise with independent draws; dQ_k = Q_k - Q_{k-1}
+    (dQ_0 = 0); rvec_k = 1 + theta_k; drvec_k = r_k - r_{k-1}; delta_t_ratio as
+    io_input.cpp:169-172.  Generator: numpy PCG64(seed).
+    """
+    theta = np.asarray(theta, np.float64)
+    N = len(theta)
+    T1 = N if T1 is None else T1
+    thk = theta if T1 == N else np.interp(np.linspace(0, N - 1, T1), np.arange(N), theta)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t1 = t_scale * thk
+    if np.any(np.diff(t1) <= 0):
+        raise ValueError("synthetic t1grid is not strictly increasing")
+
+    def field():
+        u = rng.uniform(0.5, 2.0, size=T1)
+        xi = rng.uniform(-1.0, 1.0, size=(T1, N))
+        return np.exp(-np.abs(thk[:, None] - theta[None, :]) * u[:, None]) * (1 + noise * xi)
+
+    QK, QR = field(), field()
+    dQK, dQR = np.zeros_like(QK), np.zeros_like(QR)
+    dQK[1:] = QK[1:] - QK[:-1]
+    dQR[1:] = QR[1:] - QR[:-1]
+    r = 1 + thk
+    dr = np.zeros(T1)
+    dr[1:] = r[1:] - r[:-1]
+    rho = np.zeros(T1)
+    for i in range(2, T1):
+        rho[i] = (t1[i] - t1[i - 1]) / (t1[i - 1] - t1[i - 2])
+    return dict(t1grid=t1, dtratio=rho, rvec=r, drvec=dr, QK=QK.ravel(), QR=QR.ravel(), dQK=dQK.ravel(),
+                dQR=dQR.ravel())

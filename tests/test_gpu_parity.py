@@ -1,0 +1,163 @@
+"""GPU parity: the HIP path (through the C ABI) against the C oracle on the same inputs.
+
+Bars: time positions (posB1xOld / posB2xOld) bit-exact; interpolated fields,
+self-energies and the RHS within 1e-12 relative to the field scale; one RK step
+within 1e-12 relative; free-running quench observables within 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+import dynamite_amd as dm
+from oracle_lib import Oracle, synthetic_history
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = {}
+
+
+def grid(N, Tmax=1e3, method=dm.INTERP_POLY, order=9, s=5, fh=0):
+    key = (N, Tmax, method, order, s, fh)
+    if key not in GRIDS:
+        GRIDS[key] = dm.generate_grid(N, Tmax, s, method, order, fh)
+    return GRIDS[key]
+
+
+def pair(g, Gamma=0.8, lam=0.5, p=3, p2=4, log_resp=0, state=None):
+    prm = dm.default_params(p=p, p2=p2, lambda_=lam, T0=1e50, Gamma=Gamma, log_response=log_resp)
+    ctx = dm.Context(g, prm)
+    orc = Oracle(g, p=p, p2=p2, lam=lam, T0=1e50, Gamma=Gamma, log_resp=log_resp)
+    if state is not None:
+        ctx.set_state(state["t1grid"], state["QK"], state["QR"], state["dQK"], state["dQR"], state["rvec"],
+                      state["drvec"], state["dtratio"])
+        orc.set_history(state["t1grid"], state["dtratio"], state["rvec"], state["drvec"], state["QK"],
+                        state["QR"], state["dQK"], state["dQR"])
+    return ctx, orc
+
+
+def rel_close(a, b, rtol):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max() / scale
+    assert err <= rtol, f"relative error {err:.3e} > {rtol:.1e}"
+    return err
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_bsearch_positions_bit_exact(N):
+    g = grid(N)
+    st = synthetic_history(g.theta)
+    ctx, orc = pair(g, state=st)
+    ctx.interpolate(dm.POS_BSEARCH)
+    orc.interpolate(0)
+    b1, b2 = ctx.positions()
+    o1, o2 = orc.positions()
+    assert np.array_equal(b1, o1)
+    assert np.array_equal(b2, o2)
+
+
+@pytest.mark.parametrize("N,shift", [(64, 1), (128, 3)])
+def test_isearch_positions_bit_exact(N, shift):
+    g = grid(N)
+    st = synthetic_history(g.theta)
+    # seed positions from an earlier history (last `shift` rows dropped), as after RK stages
+    T1 = len(st["t1grid"])
+    early = {k: (v[:T1 - shift] if len(v) == T1 else v[:(T1 - shift) * N]) for k, v in st.items()}
+    ctx, orc = pair(g, state=early)
+    ctx.interpolate(dm.POS_BSEARCH)
+    orc.interpolate(0)
+    ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
+    orc.set_history(st["t1grid"], st["dtratio"], st["rvec"], st["drvec"], st["QK"], st["QR"], st["dQK"], st["dQR"])
+    b1, b2 = ctx.positions()
+    ctx.set_positions(b1, b2)
+    ctx.interpolate(dm.POS_ISEARCH)
+    orc.interpolate(1)
+    g1, g2 = ctx.positions()
+    o1, o2 = orc.positions()
+    assert np.array_equal(g1, o1)
+    assert np.array_equal(g2, o2)
+
+
+@pytest.mark.parametrize("N,log_resp", [(64, 0), (128, 0), (64, 1)])
+def test_fields_and_rhs(N, log_resp):
+    g = grid(N)
+    st = synthetic_history(g.theta)
+    ctx, orc = pair(g, state=st, log_resp=log_resp)
+    ctx.interpolate(dm.POS_BSEARCH)
+    orc.interpolate(0)
+    fg, fo = ctx.fields(), orc.fields()
+    for name in dm.FIELD_NAMES + ("rInt",):
+        # log mode: the reference accumulates the log-stencil in long double (index_vec.cpp:159-172)
+        rel_close(fg[name], fo[name], 1e-12 if not log_resp else 1e-10)
+    hK, hR = ctx.rhs()
+    oK, oR = orc.rhs()
+    rel_close(hK, oK, 1e-12)
+    rel_close(hR, oR, 1e-12)
+    assert abs(ctx.rstep() - orc.rstep()) <= 1e-12 * abs(orc.rstep())
+
+
+@pytest.mark.parametrize("method", [dm.RK54, dm.SSPRK104])
+def test_rk_step_teacher_forced(method):
+    """One integrator, dt replayed from the oracle before every step (runge_kutta.cpp:19-252)."""
+    N = 64
+    g = grid(N, 1e2)
+    ctx, orc = pair(g, Gamma=0.8)
+    ctx.init_quench()
+    orc.init_quench()
+    for k in range(40):
+        dt = min(1e-5 * 1.25 ** k, 0.02)
+        ctx.set_control(dt, 1.0, 0.0, 10, method)
+        orc.set_control(dt, 1.0, 0.0, 10, method)
+        eg = ctx.rk_step()
+        eo = orc.update()
+        if k % 8 == 7 or k == 39:
+            sg, so = ctx.get_state(), orc.state()
+            assert np.array_equal(sg["t1grid"], so["t1grid"])
+            assert np.array_equal(sg["dtratio"], so["dtratio"])
+            for key in ("QK", "QR", "dQK", "dQR", "rvec", "drvec"):
+                rel_close(sg[key], so[key], 1e-11)
+            # the embedded error is a difference of O(1) stage sums: at this size it is
+            # rounding-level, so compare against the rounding scale 4*N*eps
+            assert abs(eg - eo) <= 1e-3 * abs(eo) + 4 * N * 2.2e-16, (eg, eo)
+            b1, b2 = ctx.positions()
+            o1, o2 = orc.positions()
+            assert np.array_equal(b1, o1) and np.array_equal(b2, o2)
+
+
+@pytest.mark.parametrize("N,Tmax,steps", [(64, 1e2, 300), (128, 1e4, 150)])
+def test_quench_free_running(N, Tmax, steps):
+    g = grid(N, Tmax)
+    ctx, orc = pair(g, Gamma=0.8)
+    ctx.init_quench()
+    orc.init_quench()
+    same_dt = 0
+    for k in range(steps):
+        cg, co = ctx.get_control(), orc.control()
+        same_dt += cg["delta_t"] == co["delta_t"]
+        tg = ctx.loop_iteration()
+        to = orc.loop_iteration()
+    eg, qg = ctx.observables(tg)
+    eo = orc.energy(to)
+    so = orc.state()
+    qo = so["QK"][(len(so["t1grid"]) - 1) * N]
+    assert same_dt == steps, f"dt sequences diverged ({same_dt}/{steps} equal)"
+    assert abs(eg - eo) <= 1e-9 * abs(eo)
+    assert abs(qg - qo) <= 1e-9 * abs(qo)
+    sg = ctx.get_state()
+    rel_close(sg["QK"], so["QK"], 1e-9)
+    rel_close(sg["QR"], so["QR"], 1e-9)
+
+
+def test_sparsify_matches_oracle():
+    N = 64
+    g = grid(N, 1e2)
+    ctx, orc = pair(g, Gamma=0.8)
+    ctx.init_quench()
+    orc.init_quench()
+    for _ in range(60):
+        ctx.loop_iteration()
+        orc.loop_iteration()
+    ctx.sparsify(1e-12)
+    orc.sparsify(1e-12)
+    sg, so = ctx.get_state(), orc.state()
+    assert np.array_equal(sg["t1grid"], so["t1grid"])
+    rel_close(sg["QK"], so["QK"], 1e-11)
