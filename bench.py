@@ -128,7 +128,16 @@ def main():
     ap.add_argument("--prep", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a plain HBM copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args()
+    if a.copy_probe > 0:
+        import dynamite_amd as dm
+        nb = int(a.copy_probe * 2**20)
+        ms = dm.copy_probe(nb, 10)
+        log(f"[bench] copy probe: {nb} bytes read + {nb} written, {ms * 1e3:.1f} us/launch, "
+            f"{2 * nb / ms / 1e6:.0f} GB/s")
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {a.gpus}); "
               "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
@@ -143,8 +152,9 @@ def main():
     B = algorithmic_bytes_rhs(N, g.m)
     achieved = B / avg_rhs_s / 1e9
     log(f"[bench] {ms_step:.3f} ms/step; rhs_rows {rhs_n} launches avg {avg_rhs_s * 1e6:.1f} us "
-        f"-> {achieved:.0f} GB/s algorithmic; search avg {prof[0][0] / max(prof[0][1], 1) * 1e3:.1f} us; "
-        f"finish avg {prof[2][0] / max(prof[2][1], 1) * 1e3:.1f} us")
+        f"-> {achieved:.0f} GB/s algorithmic; stage (finish+search) avg "
+        f"{prof[2][0] / max(prof[2][1], 1) * 1e3:.1f} us x {prof[2][1]}; search-only avg "
+        f"{prof[0][0] / max(prof[0][1], 1) * 1e3:.1f} us x {prof[0][1]}")
     traffic = read_traffic()
     out = {
         "metric": METRIC,

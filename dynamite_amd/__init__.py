@@ -293,3 +293,10 @@ def profile_read(ctx, kind):
     ms, n = C.c_double(), C.c_int64()
     _check(lib().dmfe_profile_read(ctx._h, C.c_int(kind), C.byref(ms), C.byref(n)), "dmfe_profile_read")
     return ms.value, n.value
+
+
+def copy_probe(nbytes, reps=10):
+    """Average ms of a plain HBM copy of nbytes (counter calibration)."""
+    ms = C.c_double()
+    _check(lib().dmfe_copy_probe(C.c_int64(int(nbytes)), C.c_int(reps), C.byref(ms)), "dmfe_copy_probe")
+    return ms.value

@@ -214,18 +214,31 @@ int launch_check(const char* what) {
     return DMFE_OK;
 }
 
-// interpolate(): positions + B1 prologue (interpolation_core.cpp:19-113)
-int run_search(dmfe_ctx* c, int mode, const double* srcB1, const double* srcB2, double* saveB1, double* saveB2) {
+SearchArgs make_search(const dmfe_ctx* c, int mode, const double* srcB1, const double* srcB2, double* saveB1,
+                       double* saveB2) {
     SearchArgs a{};
     a.mode = mode;
     a.T1 = (int)c->T1;
+    a.t1 = c->H.t1;
     a.tl = c->t1[c->T1 - 1];
     a.srcB1 = srcB1; a.srcB2 = srcB2; a.saveB1 = saveB1; a.saveB2 = saveB2;
-    prof_begin(c, kKindSearch);
-    launch_search(c->tables(), c->H, c->W, c->model, a, c->stream);
-    prof_end(c, kKindSearch);
-    c->interp_valid = true;
-    return launch_check("search_kernel");
+    return a;
+}
+
+// One stage-boundary launch: finish of the previous stage (f) and/or the searches
+// of the next interpolation (sa, interpolation_core.cpp:19-84).
+int run_stage(dmfe_ctx* c, const SearchArgs* sa, const FinishArgs* f) {
+    const int kind = f ? kKindFinish : kKindSearch;
+    prof_begin(c, kind);
+    launch_stage(c->tables(), c->H, c->W, c->model, sa, f, c->stream);
+    prof_end(c, kind);
+    if (sa) c->interp_valid = true;
+    return launch_check("stage_kernel");
+}
+
+int run_search(dmfe_ctx* c, int mode, const double* srcB1, const double* srcB2, double* saveB1, double* saveB2) {
+    const SearchArgs a = make_search(c, mode, srcB1, srcB2, saveB1, saveB2);
+    return run_stage(c, &a, nullptr);
 }
 
 int run_rhs_rows(dmfe_ctx* c, double* fields) {
@@ -247,7 +260,10 @@ int copy_out(dmfe_ctx* c, double* dst, const void* src, size_t bytes) {
     return DMFE_OK;
 }
 
-// update() = RK54() / SSPRK104() (runge_kutta.cpp:19-261).
+// update() = RK54() / SSPRK104() (runge_kutta.cpp:19-261).  Launch sequence per
+// step: [search(0) if T1 == 1] rhs(0) | finish(0)+search(1) | rhs(1) | ... |
+// rhs(S-1) | finish(S-1)+final search; the host replays the scalar time
+// bookkeeping (gt, t1grid, delta_t_ratio) in the reference's FP64 order.
 int rk_step(dmfe_ctx* c, double* err_out) {
     const bool ss = (c->method != DMFE_RK54);
     const Tableau& tb = ss ? ssprk104() : dp54();
@@ -258,30 +274,37 @@ int rk_step(dmfe_ctx* c, double* err_out) {
         rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
         if (rc) return rc;
     }
-    const Tables T = c->tables();
     const int64_t g0row = c->T1 - 1;
     const double dt = c->dt;
     double gt[kMaxStages + 1];
     double ht[kMaxStages];
     gt[0] = c->t1[c->T1 - 1];
+    FinishArgs pending{};
+    bool have_pending = false;
     for (int n = 0; n < S; ++n) {
+        SearchArgs sa{};
+        bool search = false;
         if (c->T1 == 1 || n != 0) {
-            if (n == 0) rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+            search = true;
+            if (n == 0) sa = make_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
             else if (ss && n >= 5 && n <= 7)
-                rc = run_search(c, kSearchCopy, c->W.saveB1 + (size_t)(n - 5) * c->N, c->W.saveB2 + (size_t)(n - 5) * c->P,
-                                nullptr, nullptr);
+                sa = make_search(c, kSearchCopy, c->W.saveB1 + (size_t)(n - 5) * c->N,
+                                 c->W.saveB2 + (size_t)(n - 5) * c->P, nullptr, nullptr);
             else if (ss && n >= 2 && n <= 4)
-                rc = run_search(c, kSearchI, nullptr, nullptr, c->W.saveB1 + (size_t)(n - 2) * c->N,
-                                c->W.saveB2 + (size_t)(n - 2) * c->P);
-            else rc = run_search(c, kSearchI, nullptr, nullptr, nullptr, nullptr);
+                sa = make_search(c, kSearchI, nullptr, nullptr, c->W.saveB1 + (size_t)(n - 2) * c->N,
+                                 c->W.saveB2 + (size_t)(n - 2) * c->P);
+            else sa = make_search(c, kSearchI, nullptr, nullptr, nullptr, nullptr);
+        }
+        if (search || have_pending) {
+            rc = run_stage(c, search ? &sa : nullptr, have_pending ? &pending : nullptr);
             if (rc) return rc;
-        } else if (ss && n >= 2 && n <= 4) {
-            // unreachable (n == 0); kept for symmetry with the reference's stage table
+            have_pending = false;
         }
         rc = run_rhs_rows(c, nullptr);
         if (rc) return rc;
         ht[n] = 1.0;
-        FinishArgs f{};
+        FinishArgs& f = pending;
+        f = FinishArgs{};
         f.mode = 1;
         f.stage = n;
         f.split = c->split;
@@ -333,14 +356,11 @@ int rk_step(dmfe_ctx* c, double* err_out) {
             }
             f.gt_err = std::fabs(gt[S] - gte);
         }
-        prof_begin(c, kKindFinish);
-        launch_finish(T, c->H, c->W, c->model, f, c->stream);
-        prof_end(c, kKindFinish);
-        rc = launch_check("finish_kernel");
-        if (rc) return rc;
+        have_pending = true;
     }
     // final interpolation: RK54 reuses the positions, SSPRK104 searches (:121, :229)
-    rc = run_search(c, ss ? kSearchI : kSearchSame, nullptr, nullptr, nullptr, nullptr);
+    const SearchArgs fin = make_search(c, ss ? kSearchI : kSearchSame, nullptr, nullptr, nullptr, nullptr);
+    rc = run_stage(c, &fin, &pending);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(c->pinned, c->W.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -702,6 +722,13 @@ extern "C" int dmfe_get_fields(dmfe_ctx* c, double* out) {
     if (!c->fields) HIP_TRY(hipMalloc(&c->fields, (size_t)c->P * 15 * sizeof(double)));
     int rc = run_rhs_rows(c, c->fields);
     if (rc) return rc;
+    FinishArgs f{};  // RHS epilogue only: also leaves rInt in the work buffer
+    f.mode = 0;
+    f.split = c->split;
+    f.lr = (int)(c->T1 - 1);
+    f.t_rhs = c->t1[c->T1 - 1];
+    rc = run_stage(c, nullptr, &f);
+    if (rc) return rc;
     rc = copy_out(c, out, c->fields, (size_t)c->P * 15 * sizeof(double));
     if (rc) return rc;
     return copy_out(c, out + (size_t)c->P * 15, c->W.rInt, (size_t)c->N * sizeof(double));
@@ -717,8 +744,7 @@ extern "C" int dmfe_rhs(dmfe_ctx* c, double* hK, double* hR) {
     f.split = c->split;
     f.lr = (int)(c->T1 - 1);
     f.t_rhs = c->t1[c->T1 - 1];
-    launch_finish(c->tables(), c->H, c->W, c->model, f, c->stream);
-    rc = launch_check("finish_kernel");
+    rc = run_stage(c, nullptr, &f);
     if (rc) return rc;
     rc = copy_out(c, hK, c->W.hK, (size_t)c->N * sizeof(double));
     if (rc) return rc;
@@ -803,5 +829,31 @@ extern "C" int dmfe_profile_read(dmfe_ctx* c, int kind, double* total_ms, int64_
     }
     *total_ms = tot;
     *launches = (int64_t)(n / 2);
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_copy_probe(int64_t bytes, int reps, double* ms_out) {
+    if (bytes < 16 || reps < 1) return set_error(DMFE_ERR_ARG, "dmfe_copy_probe: bad argument");
+    const long long n = bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, (size_t)n * 16));
+    HIP_TRY(hipMalloc(&b, (size_t)n * 16));
+    HIP_TRY(hipMemset(a, 0, (size_t)n * 16));
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    launch_hbm_copy(a, b, n, nullptr);
+    hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) launch_hbm_copy(a, b, n, nullptr);
+    hipEventRecord(e1, nullptr);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("dmfe_copy_probe: ") + hipGetErrorString(e));
+    if (ms_out) *ms_out = ms / reps;
     return DMFE_OK;
 }

@@ -72,6 +72,7 @@ enum SearchMode { kSearchB = 0, kSearchI = 1, kSearchSame = 2, kSearchCopy = 3 }
 struct SearchArgs {
     int mode;
     int T1;
+    const double* t1;     // t1grid in HBM; entry T1-1 is taken from tl
     double tl;            // t1grid.back()
     const double* srcB1;  // kSearchCopy source
     const double* srcB2;
@@ -105,12 +106,11 @@ struct FinishArgs {
     double gt_err;             // |gt_S - gte|
 };
 
-void launch_search(const Tables& T, const History& H, const Work& W, const Model& M, const SearchArgs& a,
-                   hipStream_t s);
+// finish of the previous stage (fa) and/or the next interpolation's searches (sa) in one launch
+void launch_stage(const Tables& T, const History& H, const Work& W, const Model& M, const SearchArgs* sa,
+                  const FinishArgs* fa, hipStream_t s);
 void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
                      hipStream_t s);
-void launch_finish(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
-                   hipStream_t s);
 // rstep (what = 0) or energy at time t (what = 1) on history row `row`, into W.scal[2]
 void launch_row_scalar(const Tables& T, const History& H, const Work& W, const Model& M, int row, double t,
                        int what, hipStream_t s);
@@ -119,5 +119,7 @@ void launch_sparsify_values(const Tables& T, const History& H, int T1, double* v
 // gather kept rows: dst row r <- src row keep[r]
 void launch_gather_rows(const Tables& T, const History& src, const History& dst, const int* keep, int nkeep,
                         hipStream_t s);
+
+void launch_hbm_copy(const double2* src, double2* dst, long long n, hipStream_t s);
 
 }  // namespace dmfe

@@ -1,16 +1,18 @@
 // gfx950 kernels of the per-RK-stage DMFT update (FP64).
 //
-//   search_kernel    time positions of theta*t (B1) and phi2*t (B2) - binary search or
-//                    the seeded interpolation search, bit-exact with search_utils.cpp:13-131 -
-//                    and the B1 prologue (Floor, diff, rInt; interpolation_core.cpp:57-113).
+//   stage_kernel     one launch per stage boundary: the finish of the previous stage
+//                    (block 0) together with the time positions of theta*t (B1) and
+//                    phi2*t (B2) of the next interpolation - binary search or the seeded
+//                    interpolation search, bit-exact with search_utils.cpp:13-131 - and
+//                    the B1 prologue (Floor, diff; interpolation_core.cpp:57-84).
 //   rhs_rows_kernel  one (row j, column chunk) per workgroup: rebuilds the A1/A2/B1/B2
 //                    interpolants of row j (index_vec.cpp:15-156, index_mat.cpp:18-91), the
 //                    self-energies (math_sigma.cpp:7-62) and the eight int.dat quadratures
 //                    of QKstep/QRstep (time_steps.cpp:27-47,94-100) without materialising
 //                    any N*N field in HBM.
-//   finish_kernel    one workgroup: RHS epilogue, RK stage combination (runge_kutta.cpp),
-//                    append/replace of the history row (time_steps.cpp:49-81,127-162),
-//                    rstep, drstep2 and the embedded error norm.
+//   finish_block     (block 0 of stage_kernel) RHS epilogue, RK stage combination
+//                    (runge_kutta.cpp), append/replace of the history row
+//                    (time_steps.cpp:49-81,127-162), rstep, drstep2 and the error norm.
 //
 // Memory layout (HBM): history as two row-major [T1][N] arrays of double2,
 // V = (QK, QR) and D = (dQK, dQR), so every gather is one 16-byte load that
@@ -82,20 +84,29 @@ __device__ inline void block_sum(double (&v)[K], double* lds /* K*NT/64 */) {
 }
 
 // ---- searches (search_utils.cpp:13-131) --------------------------------------------
-// Every list index is clamped into [0, length) so that no input can read out of
-// bounds; on the reference's own (well-formed) inputs the clamp never binds.
-#pragma clang fp contract(off)
-__device__ inline double list_at(const double* list, size_t length, size_t i) {
-    return list[i < length ? i : length - 1];
-}
+// t1grid as seen by a stage: entries [0, L-1) from HBM and the newest time (the
+// value the same launch's finish block is writing) from the kernel argument.
+// Indices are clamped into [0, L) so that no input can read out of bounds; on the
+// reference's own (well-formed) inputs the clamp never binds.
+struct ListView {
+    const double* g;
+    size_t L;
+    double last;
+    __device__ inline double operator()(size_t i) const {
+        if (i >= L - 1) return last;
+        return g[i];
+    }
+};
 
-__device__ double bsearch_elem(const double* list, size_t len, double El) {
+#pragma clang fp contract(off)
+__device__ double bsearch_elem(const ListView& list, double El) {
+    const size_t len = list.L;
     size_t n0 = 1, n1 = len, m = 1;
     double Lm = 0.0;
-    if (list[0] < El) {
+    if (list(0) < El) {
         while (n0 <= n1) {
             m = (n0 + n1) / 2;
-            Lm = list_at(list, len, m - 1);
+            Lm = list(m - 1);
             if (Lm == El) { n0 = m; break; }
             if (Lm < El) n0 = m + 1;
             else n1 = m - 1;
@@ -103,95 +114,100 @@ __device__ double bsearch_elem(const double* list, size_t len, double El) {
     }
     if (Lm <= El) {
         if (m == len) return (double)m;
-        return m + (El - Lm) / (list_at(list, len, m) - Lm);
+        return m + (El - Lm) / (list(m) - Lm);
     }
-    if (m > 1) return m - (El - Lm) / (list_at(list, len, m - 2) - Lm);
+    if (m > 1) return m - (El - Lm) / (list(m - 2) - Lm);
     return (double)m;
 }
 
 // One element of isearchPosSortedInit with its upper bracket n1 given.
-__device__ double isearch_elem(const double* list, size_t length, double El, double init, size_t n1) {
+__device__ double isearch_elem(const ListView& list, double El, double init, size_t n1) {
+    const size_t length = list.L;
     size_t n0 = (size_t)floor(init);
     if (n0 < 1) n0 = 1;  // the reference reads list[-1] here; positions are >= 1 on its inputs
     size_t m = (n0 + 1 < length) ? n0 + 1 : length;
     bool even = true;
-    double l0, l1, Lm = list_at(list, length, m - 1);
+    double l0, l1, Lm = list(m - 1);
     if (Lm > El) {
         const size_t a = m - 2;  // wraps for m < 2 exactly like the size_t code
         n1 = a > 1 ? a : 1;
     }
     int guard = 0;
-    while ((l0 = list_at(list, length, n0 - 1)) <= El && El <= (l1 = list_at(list, length, n1 - 1)) && n0 < n1 &&
-           guard++ < 512) {
+    while ((l0 = list(n0 - 1)) <= El && El <= (l1 = list(n1 - 1)) && n0 < n1 && guard++ < 512) {
         even = !even;
         if (even) m = (size_t)((double)n0 + round((El - l0) / (l1 - l0) * (double)(n1 - n0)));
         else m = (n0 + n1) / 2;
-        Lm = list_at(list, length, m - 1);
+        Lm = list(m - 1);
         if (Lm == El) { n0 = m; n1 = m - 1; }
         else if (Lm < El) n0 = m + 1;
         else n1 = m - 1;
     }
     if (Lm <= El) {
         if (m == length) return (double)m;
-        return m + (El - Lm) / (list_at(list, length, m) - Lm);
+        return m + (El - Lm) / (list(m) - Lm);
     }
-    if (m > 1) return m - (El - Lm) / (list_at(list, length, m - 2) - Lm);
+    if (m > 1) return m - (El - Lm) / (list(m - 2) - Lm);
     return (double)m;
 }
 #pragma clang fp contract(on)
 
-constexpr int kSearchThreads = 256;
-constexpr int kSearchEPT = kMaxN / kSearchThreads;
+constexpr int kStageThreads = 256;
 
-// Positions of one row of `len` elements (elements elem[l] * tl).  The reference
-// walks a row from its last element to its first and feeds ceil(result) of
-// element l+1 into element l as the upper bracket n1; results depend on the
-// previous element only through that bracket.  Here every lane searches its
-// elements with the brackets of the previous sweep until no bracket changes:
-// after k sweeps the last k elements are final, and a sweep that changes no
-// bracket reproduces the sequential walk exactly (typically 2-3 sweeps).
-__device__ void search_row(const double* list, int length, double tl, const double* elem, const double* init,
+__device__ inline size_t bracket_of(double pos, size_t L) {
+    const double cv = ceil(pos);
+    return (cv >= (double)L) ? L : (cv < 1.0 ? 1 : (size_t)cv);
+}
+
+// Positions of one row of `len` elements elem[l] * tl.  The reference walks a row
+// from its last element to its first and feeds ceil(result) of element l+1 into
+// element l as the upper bracket n1; an element's result depends on its
+// predecessor only through that bracket.  Here every lane searches its elements
+// with the brackets of the previous sweep until no bracket changes: after k
+// sweeps the last k elements are final, and a sweep that changes no bracket
+// reproduces the sequential walk exactly.  The first sweep takes its brackets
+// from the previous positions (init), which are usually already exact, so most
+// rows finish in one or two sweeps.
+template <int EPT>
+__device__ void search_row(const ListView& list, double tl, const double* __restrict__ elem, const double* init,
                            double* out, double* out2, int len, int mode, double* lds) {
     const int tid = threadIdx.x;
-    double El[kSearchEPT], ini[kSearchEPT], res[kSearchEPT];
-    size_t c[kSearchEPT];
-    const size_t L = (size_t)length;
+    double El[EPT], ini[EPT], res[EPT];
+    size_t c[EPT];
+    const size_t L = list.L;
 #pragma unroll
-    for (int k = 0; k < kSearchEPT; ++k) {
-        const int l = tid + k * kSearchThreads;
+    for (int k = 0; k < EPT; ++k) {
+        const int l = tid + k * kStageThreads;
         El[k] = 0.0; ini[k] = 1.0; res[k] = 0.0; c[k] = L;
         if (l < len) {
             El[k] = (mode == kSearchB) ? elem[l] * tl : tl * elem[l];
             ini[k] = init[l];
+            if (mode != kSearchB && l + 1 < len) c[k] = bracket_of(init[l + 1], L);
         }
     }
+    __syncthreads();  // all inits are read before any lane overwrites them (in-place rows)
     if (mode == kSearchB) {
 #pragma unroll
-        for (int k = 0; k < kSearchEPT; ++k) {
-            const int l = tid + k * kSearchThreads;
-            if (l < len) res[k] = bsearch_elem(list, L, El[k]);
+        for (int k = 0; k < EPT; ++k) {
+            const int l = tid + k * kStageThreads;
+            if (l < len) res[k] = bsearch_elem(list, El[k]);
         }
     } else {
         for (int sweep = 0; sweep <= len + 1; ++sweep) {
 #pragma unroll
-            for (int k = 0; k < kSearchEPT; ++k) {
-                const int l = tid + k * kSearchThreads;
+            for (int k = 0; k < EPT; ++k) {
+                const int l = tid + k * kStageThreads;
                 if (l < len) {
-                    res[k] = isearch_elem(list, L, El[k], ini[k], c[k]);
+                    res[k] = isearch_elem(list, El[k], ini[k], c[k]);
                     lds[l] = res[k];
                 }
             }
             __syncthreads();
             int changed = 0;
 #pragma unroll
-            for (int k = 0; k < kSearchEPT; ++k) {
-                const int l = tid + k * kSearchThreads;
+            for (int k = 0; k < EPT; ++k) {
+                const int l = tid + k * kStageThreads;
                 if (l < len) {
-                    size_t nc = L;
-                    if (l + 1 < len) {
-                        const double cv = ceil(lds[l + 1]);
-                        nc = (cv >= (double)L) ? L : (cv < 1.0 ? 1 : (size_t)cv);
-                    }
+                    const size_t nc = (l + 1 < len) ? bracket_of(lds[l + 1], L) : L;
                     changed |= (nc != c[k]);
                     c[k] = nc;
                 }
@@ -200,8 +216,8 @@ __device__ void search_row(const double* list, int length, double tl, const doub
         }
     }
 #pragma unroll
-    for (int k = 0; k < kSearchEPT; ++k) {
-        const int l = tid + k * kSearchThreads;
+    for (int k = 0; k < EPT; ++k) {
+        const int l = tid + k * kStageThreads;
         if (l < len) {
             out[l] = res[k];
             if (out2) out2[l] = res[k];
@@ -209,75 +225,62 @@ __device__ void search_row(const double* list, int length, double tl, const doub
     }
 }
 
-// Grid: N + 1 workgroups.  Blocks 0..N-1: phi2 row k (B2 positions); block N: the
-// theta row (B1 positions) followed by the B1 prologue.
-__global__ __launch_bounds__(kSearchThreads) void search_kernel(Tables T, History H, Work W, Model M, SearchArgs a) {
-    __shared__ double lds[kMaxN];
-    __shared__ double red[kSearchThreads / kWave];
+// B2 positions of phi2 row k (interpolation_core.cpp:27-29).
+template <int EPT>
+__device__ void search_phi2_row(const Tables& T, const Work& W, const SearchArgs& a, int k, double* lds) {
     const int N = T.N;
-    const int b = blockIdx.x;
-    if (b < N) {
-        double* out = W.posB2 + (size_t)b * N;
-        double* out2 = a.saveB2 ? a.saveB2 + (size_t)b * N : nullptr;
-        if (a.mode == kSearchSame) {
-            if (out2)
-                for (int l = threadIdx.x; l < N; l += kSearchThreads) out2[l] = out[l];
-            return;
-        }
-        if (a.mode == kSearchCopy) {
-            const double* src = a.srcB2 + (size_t)b * N;
-            for (int l = threadIdx.x; l < N; l += kSearchThreads) {
-                out[l] = src[l];
-                if (out2) out2[l] = src[l];
-            }
-            return;
-        }
-        search_row(H.t1, a.T1, a.tl, T.phi2 + (size_t)b * N, out, out, out2, N, a.mode, lds);
+    double* out = W.posB2 + (size_t)k * N;
+    double* out2 = a.saveB2 ? a.saveB2 + (size_t)k * N : nullptr;
+    if (a.mode == kSearchSame) {
+        if (out2)
+            for (int l = threadIdx.x; l < N; l += kStageThreads) out2[l] = out[l];
         return;
     }
-    // ---- theta row -----------------------------------------------------------------
+    if (a.mode == kSearchCopy) {
+        const double* src = a.srcB2 + (size_t)k * N;
+        for (int l = threadIdx.x; l < N; l += kStageThreads) {
+            out[l] = src[l];
+            if (out2) out2[l] = src[l];
+        }
+        return;
+    }
+    const ListView list{a.t1, (size_t)a.T1, a.tl};
+    search_row<EPT>(list, a.tl, T.phi2 + (size_t)k * N, out, out, out2, N, a.mode, lds);
+}
+
+// B1 positions of the theta row and the B1 prologue Floor / diff
+// (interpolation_core.cpp:22-24, 57-84).
+template <int EPT>
+__device__ void search_theta_row(const Tables& T, const Work& W, const SearchArgs& a, double* lds, double* red) {
+    const int N = T.N;
     double* pos = W.posB1;
     if (a.mode == kSearchCopy) {
-        for (int l = threadIdx.x; l < N; l += kSearchThreads) pos[l] = a.srcB1[l];
+        for (int l = threadIdx.x; l < N; l += kStageThreads) pos[l] = a.srcB1[l];
     } else if (a.mode != kSearchSame) {
-        search_row(H.t1, a.T1, a.tl, T.theta, pos, pos, nullptr, N, a.mode, lds);
+        const ListView list{a.t1, (size_t)a.T1, a.tl};
+        search_row<EPT>(list, a.tl, T.theta, pos, pos, nullptr, N, a.mode, lds);
     }
     __syncthreads();
     if (a.saveB1)
-        for (int l = threadIdx.x; l < N; l += kSearchThreads) a.saveB1[l] = pos[l];
+        for (int l = threadIdx.x; l < N; l += kStageThreads) a.saveB1[l] = pos[l];
     // maxCeil = ceil(max pos) - 1, at least 1 (interpolation_core.cpp:58-67)
     double mx = -INFINITY;
-    for (int l = threadIdx.x; l < N; l += kSearchThreads) mx = fmax(mx, pos[l]);
+    for (int l = threadIdx.x; l < N; l += kStageThreads) mx = fmax(mx, pos[l]);
     mx = wave_max(mx);
     if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = mx;
     __syncthreads();
     mx = red[0];
-    for (int w = 1; w < kSearchThreads / kWave; ++w) mx = fmax(mx, red[w]);
+    for (int w = 1; w < kStageThreads / kWave; ++w) mx = fmax(mx, red[w]);
     size_t maxCeil = (size_t)ceil(mx) - 1;
     if (maxCeil < 1) maxCeil = 1;
     const size_t t1len = (size_t)a.T1;
-    const bool live = a.tl > 0;
-    for (int i = threadIdx.x; i < N; i += kSearchThreads) {
+    for (int i = threadIdx.x; i < N; i += kStageThreads) {
         size_t f = (size_t)floor(pos[i]);
         if (f < 1) f = 1;
         else if (f > maxCeil) f = maxCeil;
         if (f > t1len - 1) f = t1len > 1 ? t1len - 1 : 1;  // memory guard; never binds on valid positions
-        const double d = (double)f - pos[i];
         W.Floor[i] = (int)f;
-        W.diff[i] = d;
-        // rInt: Hermite of rvec/drvec at the B1 positions (index_vec.cpp:138-156)
-        double ri;
-        if (!live) {
-            ri = H.r[0];
-        } else {
-            const double d2 = d * d, d3 = d * (d * d);
-            if (f < t1len - 1)
-                ri = (1 - 3 * d2 - 2 * d3) * H.r[f - 1] + (3 * d2 + 2 * d3) * H.r[f] - (d + 2 * d2 + d3) * H.dr[f] -
-                     (d2 + d3) * H.dr[f + 1] / H.rho[f + 1];
-            else
-                ri = (1 - d2) * H.r[f - 1] + d2 * H.r[f] - (d + d2) * H.dr[f];
-        }
-        W.rInt[i] = ri;
+        W.diff[i] = (double)f - pos[i];
     }
 }
 
@@ -291,6 +294,16 @@ __device__ inline int xcd_remap(int b, int nb) {
     constexpr int X = 8;
     if (nb % X) return b;
     return (b % X) * (nb / X) + b / X;
+}
+
+__device__ inline double rint_at(const History& H, const Work& W, int i, int T1, bool live) {
+    if (!live) return H.r[0];
+    const size_t f = (size_t)W.Floor[i], t1len = (size_t)T1;
+    const double d = W.diff[i], d2 = d * d, d3 = d * (d * d);
+    if (f < t1len - 1)
+        return (1 - 3 * d2 - 2 * d3) * H.r[f - 1] + (3 * d2 + 2 * d3) * H.r[f] - (d + 2 * d2 + d3) * H.dr[f] -
+               (d2 + d3) * H.dr[f + 1] / H.rho[f + 1];
+    return (1 - d2) * H.r[f - 1] + d2 * H.r[f] - (d + d2) * H.dr[f];
 }
 
 template <bool LOGR>
@@ -461,21 +474,22 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History
 }
 
 // ---- finish: RHS epilogue + RK stage -------------------------------------------------
-constexpr int kFinThreads = 512;
-constexpr int kFinEPT = kMaxN / kFinThreads;
-
-__global__ __launch_bounds__(kFinThreads) void finish_kernel(Tables T, History H, Work W, Model M, FinishArgs a) {
-    __shared__ double red[8 * (kFinThreads / kWave)];
+template <int EPT>
+__device__ void finish_block(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
+                             double* red /* 8 * kStageThreads / 64 */) {
     const int N = T.N;
     const int n = a.stage;
     const double t = a.t_rhs;
     const double2* last = H.V + (size_t)a.lr * N;
     const double qK0 = last[0].x;
-    const double s1 = fl1(M, qK0) / M.T0, s2 = -W.rInt[N - 1], s3 = qK0 / M.T0, s4 = 2 * M.Gamma;
-    double hk[kFinEPT], hr[kFinEPT];
+    // rInt (index_vec.cpp:138-156) from the B1 prologue of the current interpolation
+    const bool live = t > 0;
+    const double s1 = fl1(M, qK0) / M.T0, s2 = -rint_at(H, W, N - 1, a.lr + 1, live), s3 = qK0 / M.T0,
+                 s4 = 2 * M.Gamma;
+    double hk[EPT], hr[EPT];
 #pragma unroll
-    for (int e = 0; e < kFinEPT; ++e) {
-        const int j = threadIdx.x + e * kFinThreads;
+    for (int e = 0; e < EPT; ++e) {
+        const int j = threadIdx.x + e * kStageThreads;
         hk[e] = hr[e] = 0.0;
         if (j >= N) continue;
         double C[kSums];
@@ -488,7 +502,9 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(Tables T, History H
         }
         const double th = T.theta[j];
         const double fA = t * th, fR = t * (1 - th);
-        const double qK = last[j].x, qR = last[j].y, ri = W.rInt[j], b1 = W.qkb1c0[j];
+        const double ri = rint_at(H, W, j, a.lr + 1, live);
+        W.rInt[j] = ri;
+        const double qK = last[j].x, qR = last[j].y, b1 = W.qkb1c0[j];
         double d1 = b1 * s1 + qK * s2;
         d1 = d1 + C[0] * fR;
         d1 = d1 + C[1] * fA;
@@ -510,8 +526,8 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(Tables T, History H
         double v[6] = {0, 0, 0, 0, 0, 0};
         double dsK0 = 0.0, sK0 = 0.0;
 #pragma unroll
-        for (int e = 0; e < kFinEPT; ++e) {
-            const int j = threadIdx.x + e * kFinThreads;
+        for (int e = 0; e < EPT; ++e) {
+            const int j = threadIdx.x + e * kStageThreads;
             if (j >= N) continue;
             const double qK = last[j].x, qR = last[j].y, w = T.integ[j];
             const double sK = fl1(M, qK), sR = fl2(M, qK) * qR;
@@ -525,7 +541,7 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(Tables T, History H
             v[5] += (t * w) * sK * hr[e];
             if (j == 0) { dsK0 = dsK; sK0 = sK; }
         }
-        block_sum<6, kFinThreads>(v, red);
+        block_sum<6, kStageThreads>(v, red);
         if (threadIdx.x == 0)
             W.scal[0] = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + (dsK0 * qK0 + sK0 * hk[0]) / M.T0;
         __syncthreads();
@@ -539,8 +555,8 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(Tables T, History H
     double err[1] = {0.0};
     double gK0 = 0.0, sKg0 = 0.0;
 #pragma unroll
-    for (int e = 0; e < kFinEPT; ++e) {
-        const int j = threadIdx.x + e * kFinThreads;
+    for (int e = 0; e < EPT; ++e) {
+        const int j = threadIdx.x + e * kStageThreads;
         if (j >= N) continue;
         const double2 g = g0[j];
         double gK = g.x, gR = g.y;
@@ -571,8 +587,8 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(Tables T, History H
             err[0] += fabs(gK - eK) + fabs(gR - eR);
         }
     }
-    block_sum<2, kFinThreads>(rr, red);
-    if (a.last) block_sum<1, kFinThreads>(err, red);
+    block_sum<2, kStageThreads>(rr, red);
+    if (a.last) block_sum<1, kStageThreads>(err, red);
     if (threadIdx.x == 0) {
         H.t1[a.wrow] = a.t_new;
         H.rho[a.wrow] = a.rho_new;
@@ -582,14 +598,45 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(Tables T, History H
     }
 }
 
+// ---- one launch per stage boundary ------------------------------------------------
+// Block 0 (optional): finish of the previous stage.  Then (optional) the theta-row
+// search and the N phi2-row searches of the next interpolation.  The searches read
+// t1grid through ListView, whose newest entry comes from the argument, so they do
+// not depend on the finish block's writes; everything that does (the RHS kernel)
+// is launched afterwards.
+enum StageFlags { kHasFinish = 1, kHasTheta = 2, kHasRows = 4 };
+
+template <int EPT>
+__global__ __launch_bounds__(kStageThreads) void stage_kernel(Tables T, History H, Work W, Model M, SearchArgs sa,
+                                                              FinishArgs fa, int flags) {
+    __shared__ double lds[EPT * kStageThreads];
+    __shared__ double red[8 * (kStageThreads / kWave)];
+    int b = blockIdx.x;
+    if (flags & kHasFinish) {
+        if (b == 0) {
+            finish_block<EPT>(T, H, W, M, fa, red);
+            return;
+        }
+        --b;
+    }
+    if (flags & kHasTheta) {
+        if (b == 0) {
+            search_theta_row<EPT>(T, W, sa, lds, red);
+            return;
+        }
+        --b;
+    }
+    if ((flags & kHasRows) && b < T.N) search_phi2_row<EPT>(T, W, sa, b, lds);
+}
+
 // rstep (what 0) / energy (what 1) on one history row
-__global__ __launch_bounds__(kFinThreads) void row_scalar_kernel(Tables T, History H, Work W, Model M, int row,
-                                                                 double t, int what) {
-    __shared__ double red[2 * (kFinThreads / kWave)];
+__global__ __launch_bounds__(kStageThreads) void row_scalar_kernel(Tables T, History H, Work W, Model M, int row,
+                                                                   double t, int what) {
+    __shared__ double red[2 * (kStageThreads / kWave)];
     const int N = T.N;
     const double2* v = H.V + (size_t)row * N;
     double s[2] = {0.0, 0.0};
-    for (int j = threadIdx.x; j < N; j += kFinThreads) {
+    for (int j = threadIdx.x; j < N; j += kStageThreads) {
         const double qK = v[j].x, qR = v[j].y, w = T.integ[j];
         const double sK = fl1(M, qK);
         if (what == 0) {
@@ -599,7 +646,7 @@ __global__ __launch_bounds__(kFinThreads) void row_scalar_kernel(Tables T, Histo
             s[0] += t * w * sK * qR;
         }
     }
-    block_sum<2, kFinThreads>(s, red);
+    block_sum<2, kStageThreads>(s, red);
     if (threadIdx.x == 0) {
         const double qK0 = v[0].x;
         if (what == 0) W.scal[2] = M.Gamma + s[0] + s[1] + fl1(M, qK0) * qK0 / M.T0;
@@ -647,11 +694,38 @@ __global__ __launch_bounds__(kRowThreads) void gather_rows_kernel(Tables T, Hist
     }
 }
 
+// Plain streaming copy, used to calibrate HBM byte counters (FETCH_SIZE / WRITE_SIZE).
+__global__ __launch_bounds__(256) void hbm_copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                       long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
-void launch_search(const Tables& T, const History& H, const Work& W, const Model& M, const SearchArgs& a,
-                   hipStream_t s) {
-    hipLaunchKernelGGL(search_kernel, dim3(T.N + 1), dim3(kSearchThreads), 0, s, T, H, W, M, a);
+void launch_hbm_copy(const double2* src, double2* dst, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(hbm_copy_kernel, dim3(2048), dim3(256), 0, s, src, dst, n);
+}
+
+void launch_stage(const Tables& T, const History& H, const Work& W, const Model& M, const SearchArgs* sa,
+                  const FinishArgs* fa, hipStream_t s) {
+    int flags = 0, nb = 0;
+    SearchArgs S{};
+    FinishArgs F{};
+    if (fa) { flags |= kHasFinish; F = *fa; ++nb; }
+    if (sa) {
+        S = *sa;
+        flags |= kHasTheta;
+        ++nb;
+        const bool rows = !(sa->mode == kSearchSame && !sa->saveB2);
+        if (rows) { flags |= kHasRows; nb += T.N; }
+    }
+    if (nb == 0) return;
+    const int ept = (T.N + kStageThreads - 1) / kStageThreads;
+    if (ept <= 1) hipLaunchKernelGGL(stage_kernel<1>, dim3(nb), dim3(kStageThreads), 0, s, T, H, W, M, S, F, flags);
+    else if (ept <= 2) hipLaunchKernelGGL(stage_kernel<2>, dim3(nb), dim3(kStageThreads), 0, s, T, H, W, M, S, F, flags);
+    else if (ept <= 4) hipLaunchKernelGGL(stage_kernel<4>, dim3(nb), dim3(kStageThreads), 0, s, T, H, W, M, S, F, flags);
+    else hipLaunchKernelGGL(stage_kernel<8>, dim3(nb), dim3(kStageThreads), 0, s, T, H, W, M, S, F, flags);
 }
 
 void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
@@ -664,14 +738,9 @@ void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Mod
         hipLaunchKernelGGL(rhs_rows_kernel<false>, grid, dim3(kRowThreads), lds, s, T, H, W, M, a);
 }
 
-void launch_finish(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
-                   hipStream_t s) {
-    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kFinThreads), 0, s, T, H, W, M, a);
-}
-
 void launch_row_scalar(const Tables& T, const History& H, const Work& W, const Model& M, int row, double t,
                        int what, hipStream_t s) {
-    hipLaunchKernelGGL(row_scalar_kernel, dim3(1), dim3(kFinThreads), 0, s, T, H, W, M, row, t, what);
+    hipLaunchKernelGGL(row_scalar_kernel, dim3(1), dim3(kStageThreads), 0, s, T, H, W, M, row, t, what);
 }
 
 void launch_sparsify_values(const Tables& T, const History& H, int T1, double* val, hipStream_t s) {

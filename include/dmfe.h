@@ -207,6 +207,11 @@ int dmfe_sparsify(dmfe_ctx* ctx, double threshold);
 int dmfe_profile(dmfe_ctx* ctx, int enable);
 int dmfe_profile_read(dmfe_ctx* ctx, int kind, double* total_ms, int64_t* launches);
 
+/* Measurement utility (no counterpart in the reference): `reps` launches of a
+ * plain device-to-device copy of `bytes` bytes (hbm_copy_kernel), used to
+ * calibrate HBM byte counters; *ms_out = average launch time in ms. */
+int dmfe_copy_probe(int64_t bytes, int reps, double* ms_out);
+
 /* Waits for all work queued on the context's stream. */
 int dmfe_synchronize(dmfe_ctx* ctx);
 
