@@ -132,6 +132,7 @@ struct dmfe_ctx {
     int64_t loop = 0;
     int method = DMFE_RK54;
     bool interp_valid = false;
+    int fcur = 0;  // Floor/diff buffer of the latest interpolation
     // optional per-kernel timing with HIP events on the context stream
     bool profiling = false;
     std::vector<hipEvent_t> ev[kKinds];  // start/stop pairs
@@ -222,6 +223,7 @@ SearchArgs make_search(const dmfe_ctx* c, int mode, const double* srcB1, const d
     a.t1 = c->H.t1;
     a.tl = c->t1[c->T1 - 1];
     a.srcB1 = srcB1; a.srcB2 = srcB2; a.saveB1 = saveB1; a.saveB2 = saveB2;
+    a.fbuf = 1 - c->fcur;
     return a;
 }
 
@@ -229,10 +231,12 @@ SearchArgs make_search(const dmfe_ctx* c, int mode, const double* srcB1, const d
 // of the next interpolation (sa, interpolation_core.cpp:19-84).
 int run_stage(dmfe_ctx* c, const SearchArgs* sa, const FinishArgs* f) {
     const int kind = f ? kKindFinish : kKindSearch;
+    FinishArgs fl{};
+    if (f) { fl = *f; fl.fbuf = c->fcur; }
     prof_begin(c, kind);
-    launch_stage(c->tables(), c->H, c->W, c->model, sa, f, c->stream);
+    launch_stage(c->tables(), c->H, c->W, c->model, sa, f ? &fl : nullptr, c->stream);
     prof_end(c, kind);
-    if (sa) c->interp_valid = true;
+    if (sa) { c->interp_valid = true; c->fcur = sa->fbuf; }
     return launch_check("stage_kernel");
 }
 
@@ -248,6 +252,7 @@ int run_rhs_rows(dmfe_ctx* c, double* fields) {
     a.split = c->split;
     a.log_response = c->params.log_response;
     a.fields = fields;
+    a.fbuf = c->fcur;
     prof_begin(c, kKindRhs);
     launch_rhs_rows(c->tables(), c->H, c->W, c->model, a, c->stream);
     prof_end(c, kKindRhs);
@@ -542,7 +547,7 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
              hipMalloc(&c->w[f], bP * (size_t)m) == hipSuccess;
     ok = ok && hipMalloc(&c->W.posB1, bN) == hipSuccess && hipMalloc(&c->W.posB2, bP) == hipSuccess &&
          hipMalloc(&c->W.saveB1, 3 * bN) == hipSuccess && hipMalloc(&c->W.saveB2, 3 * bP) == hipSuccess &&
-         hipMalloc(&c->W.Floor, (size_t)N * sizeof(int)) == hipSuccess && hipMalloc(&c->W.diff, bN) == hipSuccess &&
+         hipMalloc(&c->W.Floor, 2 * (size_t)N * sizeof(int)) == hipSuccess && hipMalloc(&c->W.diff, 2 * bN) == hipSuccess &&
          hipMalloc(&c->W.rInt, bN) == hipSuccess &&
          hipMalloc(&c->W.partial, bN * (size_t)c->split * kSums) == hipSuccess &&
          hipMalloc(&c->W.qkb1c0, bN) == hipSuccess && hipMalloc(&c->W.hK, bN * kMaxStages) == hipSuccess &&
@@ -556,12 +561,16 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
               hipMemcpy(c->integ, g->integ, bN, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(c->phi2, g->phi2, bP, hipMemcpyHostToDevice) == hipSuccess;
     for (int f = 0; up && f < 3; ++f) {
+        // A1/A2 are read one target per lane: transposed [m][P] so a wavefront's taps are
+        // contiguous; B2 is read one tap per lane: kept entry-major [P][m] as on disk.
         for (long long k = 0; k < P; ++k) {
             it[(size_t)k] = g->inds[f][k] < 0 ? 0 : (int)g->inds[f][k];
-            for (int d = 0; d < m; ++d) wt[(size_t)d * P + k] = g->weights[f][(size_t)k * m + d];
+            if (f < 2)
+                for (int d = 0; d < m; ++d) wt[(size_t)d * P + k] = g->weights[f][(size_t)k * m + d];
         }
+        const double* src = (f < 2) ? wt.data() : g->weights[f];
         up = hipMemcpy(c->inds[f], it.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemcpy(c->w[f], wt.data(), bP * (size_t)m, hipMemcpyHostToDevice) == hipSuccess;
+             hipMemcpy(c->w[f], src, bP * (size_t)m, hipMemcpyHostToDevice) == hipSuccess;
     }
     // posB1xOld = 1, posB2xOld = 0 (initialization.cpp:234-235)
     std::vector<double> ones((size_t)N, 1.0), zeros((size_t)P, 0.0);

@@ -40,7 +40,7 @@ struct Tables {
     const double* integ;
     const double* phi2;
     const int* inds[3];    // stencil starts A1, A2, B2 [P]
-    const double* w[3];    // weights A1, A2, B2, transposed to [m][P]
+    const double* w[3];    // weights: A1, A2 transposed to [m][P]; B2 entry-major [P][m]
 };
 
 struct History {
@@ -57,8 +57,8 @@ struct Work {
     double* posB2;    // [P]  posB2xOld
     double* saveB1;   // [3][N] SSPRK stage positions
     double* saveB2;   // [3][P]
-    int* Floor;       // [N]  B1 time rows
-    double* diff;     // [N]  Floor - posB1
+    int* Floor;       // [2][N]  B1 time rows (double-buffered: a launch's search writes
+    double* diff;     // [2][N]  one buffer while its finish block reads the other)
     double* rInt;     // [N]
     double* partial;  // [N][split][kSums]
     double* qkb1c0;   // [N]  QKB1[j, 0]
@@ -78,6 +78,7 @@ struct SearchArgs {
     const double* srcB2;
     double* saveB1;       // optional second destination
     double* saveB2;
+    int fbuf;             // Floor/diff buffer written
 };
 
 struct RhsArgs {
@@ -86,6 +87,7 @@ struct RhsArgs {
     int split;        // column chunks per row
     int log_response;
     double* fields;   // non-null: materialise the 15 fields (debug)
+    int fbuf;         // Floor/diff buffer of the current interpolation
 };
 
 struct FinishArgs {
@@ -104,6 +106,7 @@ struct FinishArgs {
     int last;                  // last stage: also the error norm
     double coefE[kMaxStages];  // dt * b2_j
     double gt_err;             // |gt_S - gte|
+    int fbuf;                  // Floor/diff buffer of the interpolation this RHS used
 };
 
 // finish of the previous stage (fa) and/or the next interpolation's searches (sa) in one launch

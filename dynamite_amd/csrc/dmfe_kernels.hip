@@ -63,6 +63,26 @@ __device__ inline double wave_max(double v) {
     return v;
 }
 
+// Sum over aligned groups of G lanes (G in 4, 8, 16, 32, 64) with DPP lane swizzles
+// (quad_perm, row_half_mirror, row_mirror) and a cross-row shuffle for G >= 32.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int G>
+__device__ inline double group_sum(double v) {
+    v += dpp_f64<0xB1>(v);  // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
+    if (G >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror: lane i <-> 7-i within 8
+    if (G >= 16) v += dpp_f64<0x140>(v);  // row_mirror: lane i <-> 15-i within 16
+    if (G >= 32) v += __shfl_xor(v, 16, kWave);
+    if (G >= 64) v += __shfl_xor(v, 32, kWave);
+    return v;
+}
+
 // Sums K values over the block; every thread gets the totals in out[]. Fixed order.
 template <int K, int NT>
 __device__ inline void block_sum(double (&v)[K], double* lds /* K*NT/64 */) {
@@ -279,8 +299,8 @@ __device__ void search_theta_row(const Tables& T, const Work& W, const SearchArg
         if (f < 1) f = 1;
         else if (f > maxCeil) f = maxCeil;
         if (f > t1len - 1) f = t1len > 1 ? t1len - 1 : 1;  // memory guard; never binds on valid positions
-        W.Floor[i] = (int)f;
-        W.diff[i] = (double)f - pos[i];
+        W.Floor[(size_t)a.fbuf * N + i] = (int)f;
+        W.diff[(size_t)a.fbuf * N + i] = (double)f - pos[i];
     }
 }
 
@@ -296,16 +316,17 @@ __device__ inline int xcd_remap(int b, int nb) {
     return (b % X) * (nb / X) + b / X;
 }
 
-__device__ inline double rint_at(const History& H, const Work& W, int i, int T1, bool live) {
+__device__ inline double rint_at(const History& H, const int* Floor, const double* diff, int i, int T1, bool live) {
     if (!live) return H.r[0];
-    const size_t f = (size_t)W.Floor[i], t1len = (size_t)T1;
-    const double d = W.diff[i], d2 = d * d, d3 = d * (d * d);
+    const size_t f = (size_t)Floor[i], t1len = (size_t)T1;
+    const double d = diff[i], d2 = d * d, d3 = d * (d * d);
     if (f < t1len - 1)
         return (1 - 3 * d2 - 2 * d3) * H.r[f - 1] + (3 * d2 + 2 * d3) * H.r[f] - (d + 2 * d2 + d3) * H.dr[f] -
                (d2 + d3) * H.dr[f + 1] / H.rho[f + 1];
     return (1 - d2) * H.r[f - 1] + d2 * H.r[f] - (d + d2) * H.dr[f];
 }
 
+#pragma clang fp contract(off)  // the compensated sum below needs unfused products
 template <bool LOGR>
 __device__ inline double2 stencil_last(const double* __restrict__ w, long long P, long long k, int m, int s,
                                        const double2* __restrict__ vl) {
@@ -319,22 +340,44 @@ __device__ inline double2 stencil_last(const double* __restrict__ w, long long P
         }
         return make_double2(a, b);
     }
-    double lin = 0.0, lsum = 0.0;
+    // The reference accumulates sum_d w_d log(QR) in 80-bit long double
+    // (index_vec.cpp:159-172); a compensated dot product (exact FMA products and
+    // two-sum accumulation, Ogita-Rump-Oishi "Dot2") reproduces that accuracy in FP64.
+    double lin = 0.0, ls = 0.0, lc = 0.0;
     bool invalid = false;
     for (int d = 0; d < m; ++d) {
         const double wd = w[(long long)d * P + k];
         const double2 v = vl[s + d];
         a = a + wd * v.x;
         lin += wd * v.y;
-        if (v.y > 0.0) lsum += wd * log(v.y);
-        else invalid = true;
+        if (v.y > 0.0) {
+            const double lv = log(v.y);
+            const double p = wd * lv;
+            const double pe = fma(wd, lv, -p);
+            const double t = ls + p;
+            const double bv = t - ls;
+            lc += ((ls - (t - bv)) + (p - bv)) + pe;
+            ls = t;
+        } else {
+            invalid = true;
+        }
     }
-    return make_double2(a, invalid ? lin : exp(lsum));
+    return make_double2(a, invalid ? lin : exp(ls + lc));
 }
+#pragma clang fp contract(on)
 
-template <bool LOGR>
+// Row j, columns [c0, c1): the eight quadratures of QKstep/QRstep.
+//  phase 1 (B2, index_mat.cpp:18-91): one group of G lanes per target entry, lane d
+//          holding stencil tap d, so the four history-row gathers of a wave touch a
+//          few contiguous cache lines instead of one line per lane; the m-tap inner
+//          products are closed by an xor-shuffle reduction inside the group and the
+//          time Hermite is applied by the group's first lane.  Results go to LDS.
+//  phase 2 (lane per entry): A1/A2 stencils on the last row (LDS), the B1 time
+//          Hermite (index_vec.cpp:63-136), the self-energies and the weighted
+//          products, then one block reduction.
+template <bool LOGR, int G>
 __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History H, Work W, Model M, RhsArgs a) {
-    extern __shared__ double2 vlast[];  // last history row, N entries
+    extern __shared__ double2 dyn[];  // [N] last history row, [cw] (QKB2, QRB2), [cw] u, [cw] ix, s
     __shared__ double red[kSums * (kRowThreads / kWave)];
     const int N = T.N, m = T.m;
     const long long P = T.P;
@@ -346,13 +389,94 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History
     const int c0 = chunk * cw, c1 = min(N, c0 + cw);
     const int T1 = a.T1;
     const bool live = a.tl > 0;
+    double2* vlast = dyn;
+    double2* b2 = dyn + N;
+    double* b2u = reinterpret_cast<double*>(dyn + N + cw);
+    int* b2ix = reinterpret_cast<int*>(b2u + cw);
+    int* b2s = b2ix + cw;
     const double2* Vl = H.V + (size_t)(T1 - 1) * N;
     for (int l = threadIdx.x; l < N; l += kRowThreads) vlast[l] = Vl[l];
+
+    // ---- phase 1: B2 ----------------------------------------------------------------
+    const double c00 = H.V[0].x, c01 = H.V[0].y;
+    const int C = c1 - c0;
+    if (live) {
+        // 1a: per entry (lane per entry, coalesced): time row, stencil start, Hermite abscissa
+        size_t cap_hi = (size_t)(W.posB2[P - 1] - 0.5);
+        if (cap_hi > (size_t)(T1 - 1)) cap_hi = (size_t)(T1 - 1);  // memory guard
+        for (int e = threadIdx.x; e < C; e += kRowThreads) {
+            const long long k = (long long)j * N + c0 + e;
+            const double px = W.posB2[k];
+            size_t ix = (size_t)px;
+            if (cap_hi < ix) ix = cap_hi;
+            if (ix < 1) ix = 1;
+            b2ix[e] = (int)ix;
+            b2s[e] = T.inds[2][k];
+            b2u[e] = px - ix;
+        }
+        __syncthreads();
+        // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
+        // instruction touch 64/G entries x a few contiguous cache lines
+        constexpr int GPW = kWave / G;
+        const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+        const int g = lane / G, d0 = lane % G;
+        constexpr int kWaves = kRowThreads / kWave;
+        for (int e0 = wid * GPW; e0 < C; e0 += GPW * kWaves) {
+            const int e = e0 + g;
+            const bool ev = e < C;
+            const int ee = ev ? e : 0;
+            const size_t ix = (size_t)b2ix[ee];
+            const int sst = b2s[ee];
+            const bool b2int = ix < (size_t)(T1 - 1);
+            const double2* R0 = H.V + (ix - 1) * N + sst;
+            const double2* R1 = H.V + ix * N + sst;
+            const double2* S1 = H.D + ix * N + sst;
+            const double2* S2 = H.D + (b2int ? ix + 1 : ix) * N + sst;
+            const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
+            double acc8[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc8[q] = 0.0;
+            if (ev) {
+                for (int dd = d0; dd < m; dd += G) {
+                    const double wd = wk[dd];
+                    const double2 v0 = R0[dd], v1 = R1[dd], g1 = S1[dd], g2 = S2[dd];
+                    acc8[0] += wd * v0.x; acc8[1] += wd * v0.y;
+                    acc8[2] += wd * v1.x; acc8[3] += wd * v1.y;
+                    acc8[4] += wd * g1.x; acc8[5] += wd * g1.y;
+                    acc8[6] += wd * g2.x; acc8[7] += wd * g2.y;
+                }
+            }
+            const double y0k = group_sum<G>(acc8[0]), y0r = group_sum<G>(acc8[1]);
+            const double y1k = group_sum<G>(acc8[2]), y1r = group_sum<G>(acc8[3]);
+            const double e1k = group_sum<G>(acc8[4]), e1r = group_sum<G>(acc8[5]);
+            const double e2k = group_sum<G>(acc8[6]), e2r = group_sum<G>(acc8[7]);
+            if (d0 == 0 && ev) {
+                const double u = b2u[e], u2 = u * u, u3 = u2 * u;
+                double qk, qr;
+                if (b2int) {
+                    const double rr = H.rho[ix + 1];
+                    const double h0 = 1 - 3 * u2 + 2 * u3, h1 = u - 2 * u2 + u3, h2 = 3 * u2 - 2 * u3, h3 = -u2 + u3;
+                    qk = h0 * y0k + h1 * e1k + h2 * y1k + h3 * e2k / rr;
+                    if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
+                        qr = h0 * y0r + h1 * e1r + h2 * y1r + h3 * e2r / rr;
+                    else
+                        qr = exp(h0 * log(y0r) + h1 * (e1r / y0r) + h2 * log(y1r) + (h3 / rr) * (e2r / y1r));
+                } else {
+                    qk = (1 - u2) * y0k + u2 * y1k + (u - u2) * e1k;
+                    if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
+                        qr = (1 - u2) * y0r + u2 * y1r + (u - u2) * e1r;
+                    else
+                        qr = exp((1 - u2) * log(y0r) + u2 * log(y1r) + (u - u2) * (e1r / y0r));
+                }
+                b2[e] = make_double2(qk, qr);
+            }
+        }
+    }
     __syncthreads();
 
-    // B1 row data (row-uniform): Floor / diff of theta_j * t
-    const int F = W.Floor[j];
-    const double dB = W.diff[j];
+    // ---- phase 2: A1, A2, B1, self-energies, quadrature products -------------------------
+    const int F = W.Floor[(size_t)a.fbuf * N + j];
+    const double dB = W.diff[(size_t)a.fbuf * N + j];
     const double in3 = dB * dB, in4 = in3 * dB;
     const bool b1int = F < T1 - 1;
     const double b1a = 1 - 3 * in3 - 2 * in4, b1b = 3 * in3 + 2 * in4, b1c = dB + 2 * in3 + in4, b1d = in3 + in4;
@@ -361,10 +485,6 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History
     const double2* VF1 = H.V + (size_t)F * N;
     const double2* DF1 = H.D + (size_t)F * N;
     const double2* DF2 = H.D + (size_t)(b1int ? F + 1 : F) * N;
-    // B2 time clamp from the last position of the whole phi2 block (index_mat.cpp:33)
-    size_t cap_hi = (size_t)(W.posB2[P - 1] - 0.5);
-    if (cap_hi > (size_t)(T1 - 1)) cap_hi = (size_t)(T1 - 1);  // memory guard
-    const double c00 = H.V[0].x, c01 = H.V[0].y;
 
     double acc[kSums];
 #pragma unroll
@@ -377,11 +497,9 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History
             qkA1 = qkA2 = qkB1 = qkB2 = c00;
             qrA1 = qrA2 = qrB1 = qrB2 = c01;
         } else {
-            // A1 / A2: stencils on the last row (index_vec.cpp:15-61)
             const double2 a1 = stencil_last<LOGR>(T.w[0], P, k, m, T.inds[0][k], vlast);
             const double2 a2 = stencil_last<LOGR>(T.w[1], P, k, m, T.inds[1][k], vlast);
             qkA1 = a1.x; qrA1 = a1.y; qkA2 = a2.x; qrA2 = a2.y;
-            // B1: cubic Hermite in time along column i (index_vec.cpp:63-136)
             const double2 y0 = VF0[i], y1 = VF1[i], dy1 = DF1[i];
             if (b1int) {
                 const double2 dy2 = DF2[i];
@@ -400,50 +518,8 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History
                 else
                     qrB1 = exp((1 - in3) * log(y0.y) + in3 * log(y1.y) - (dB + in3) * (dy1.y / y0.y));
             }
-            // B2: theta stencil on history rows ix-1, ix, ix+1 + Hermite (index_mat.cpp:18-91)
-            const double px = W.posB2[k];
-            size_t ix = (size_t)px;
-            if (cap_hi < ix) ix = cap_hi;
-            if (ix < 1) ix = 1;
-            const double u = px - ix, u2 = u * u, u3 = u2 * u;
-            const int s = T.inds[2][k];
-            const double2* R0 = H.V + (size_t)(ix - 1) * N + s;
-            const double2* R1 = H.V + (size_t)ix * N + s;
-            const double2* S1 = H.D + (size_t)ix * N + s;
-            const double* wb = T.w[2] + k;
-            const bool b2int = ix < (size_t)(T1 - 1);
-            double y0k = 0, y0r = 0, y1k = 0, y1r = 0, e1k = 0, e1r = 0, e2k = 0, e2r = 0;
-            if (b2int) {
-                const double2* S2 = H.D + (size_t)(ix + 1) * N + s;
-                for (int d = 0; d < m; ++d) {
-                    const double wd = wb[(long long)d * P];
-                    const double2 v0 = R0[d], v1 = R1[d], g1 = S1[d], g2 = S2[d];
-                    y0k = y0k + wd * v0.x; y0r = y0r + wd * v0.y;
-                    y1k = y1k + wd * v1.x; y1r = y1r + wd * v1.y;
-                    e1k = e1k + wd * g1.x; e1r = e1r + wd * g1.y;
-                    e2k = e2k + wd * g2.x; e2r = e2r + wd * g2.y;
-                }
-                const double rr = H.rho[ix + 1];
-                const double h0 = 1 - 3 * u2 + 2 * u3, h1 = u - 2 * u2 + u3, h2 = 3 * u2 - 2 * u3, h3 = -u2 + u3;
-                qkB2 = h0 * y0k + h1 * e1k + h2 * y1k + h3 * e2k / rr;
-                if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
-                    qrB2 = h0 * y0r + h1 * e1r + h2 * y1r + h3 * e2r / rr;
-                else
-                    qrB2 = exp(h0 * log(y0r) + h1 * (e1r / y0r) + h2 * log(y1r) + (h3 / rr) * (e2r / y1r));
-            } else {
-                for (int d = 0; d < m; ++d) {
-                    const double wd = wb[(long long)d * P];
-                    const double2 v0 = R0[d], v1 = R1[d], g1 = S1[d];
-                    y0k = y0k + wd * v0.x; y0r = y0r + wd * v0.y;
-                    y1k = y1k + wd * v1.x; y1r = y1r + wd * v1.y;
-                    e1k = e1k + wd * g1.x; e1r = e1r + wd * g1.y;
-                }
-                qkB2 = (1 - u2) * y0k + u2 * y1k + (u - u2) * e1k;
-                if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
-                    qrB2 = (1 - u2) * y0r + u2 * y1r + (u - u2) * e1r;
-                else
-                    qrB2 = exp((1 - u2) * log(y0r) + u2 * log(y1r) + (u - u2) * (e1r / y0r));
-            }
+            const double2 vb2 = b2[i - c0];
+            qkB2 = vb2.x; qrB2 = vb2.y;
         }
         // self-energies (math_sigma.cpp:7-62)
         const double sKA1 = fl1(M, qkA1), sRA1 = fl2(M, qkA1) * qrA1;
@@ -484,7 +560,9 @@ __device__ void finish_block(const Tables& T, const History& H, const Work& W, c
     const double qK0 = last[0].x;
     // rInt (index_vec.cpp:138-156) from the B1 prologue of the current interpolation
     const bool live = t > 0;
-    const double s1 = fl1(M, qK0) / M.T0, s2 = -rint_at(H, W, N - 1, a.lr + 1, live), s3 = qK0 / M.T0,
+    const int* Fl = W.Floor + (size_t)a.fbuf * N;
+    const double* Df = W.diff + (size_t)a.fbuf * N;
+    const double s1 = fl1(M, qK0) / M.T0, s2 = -rint_at(H, Fl, Df, N - 1, a.lr + 1, live), s3 = qK0 / M.T0,
                  s4 = 2 * M.Gamma;
     double hk[EPT], hr[EPT];
 #pragma unroll
@@ -502,7 +580,7 @@ __device__ void finish_block(const Tables& T, const History& H, const Work& W, c
         }
         const double th = T.theta[j];
         const double fA = t * th, fR = t * (1 - th);
-        const double ri = rint_at(H, W, j, a.lr + 1, live);
+        const double ri = rint_at(H, Fl, Df, j, a.lr + 1, live);
         W.rInt[j] = ri;
         const double qK = last[j].x, qR = last[j].y, b1 = W.qkb1c0[j];
         double d1 = b1 * s1 + qK * s2;
@@ -731,11 +809,19 @@ void launch_stage(const Tables& T, const History& H, const Work& W, const Model&
 void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
                      hipStream_t s) {
     const dim3 grid((unsigned)(T.N * a.split));
-    const size_t lds = (size_t)T.N * sizeof(double2);
-    if (a.log_response)
-        hipLaunchKernelGGL(rhs_rows_kernel<true>, grid, dim3(kRowThreads), lds, s, T, H, W, M, a);
-    else
-        hipLaunchKernelGGL(rhs_rows_kernel<false>, grid, dim3(kRowThreads), lds, s, T, H, W, M, a);
+    const int cw = (T.N + a.split - 1) / a.split;
+    const size_t lds = (size_t)(T.N + cw) * sizeof(double2) + (size_t)cw * (sizeof(double) + 2 * sizeof(int));
+#define DMFE_RHS(LOG, G) hipLaunchKernelGGL((rhs_rows_kernel<LOG, G>), grid, dim3(kRowThreads), lds, s, T, H, W, M, a)
+    if (a.log_response) {
+        if (T.m <= 16) DMFE_RHS(true, 8);
+        else if (T.m <= 32) DMFE_RHS(true, 16);
+        else DMFE_RHS(true, 64);
+    } else {
+        if (T.m <= 16) DMFE_RHS(false, 8);
+        else if (T.m <= 32) DMFE_RHS(false, 16);
+        else DMFE_RHS(false, 64);
+    }
+#undef DMFE_RHS
 }
 
 void launch_row_scalar(const Tables& T, const History& H, const Work& W, const Model& M, int row, double t,

@@ -85,9 +85,13 @@ def test_fields_and_rhs(N, log_resp):
     ctx.interpolate(dm.POS_BSEARCH)
     orc.interpolate(0)
     fg, fo = ctx.fields(), orc.fields()
-    for name in dm.FIELD_NAMES + ("rInt",):
-        # log mode: the reference accumulates the log-stencil in long double (index_vec.cpp:159-172)
-        rel_close(fg[name], fo[name], 1e-12 if not log_resp else 1e-10)
+    errs = {name: np.abs(fg[name] - fo[name]).max() / np.abs(fo[name]).max() for name in dm.FIELD_NAMES + ("rInt",)}
+    # Log mode (-R): the B2 log-Hermite divides stencil sums by stencil sums
+    # (index_mat.cpp:52-63), so the reduction-order difference of the m-tap sums
+    # (~Lebesgue * eps ~ 1e-13 on this noisy synthetic history, see QKB2) is amplified
+    # by 1/QR_base; measured 1.3e-10 on QRB2.  Linear mode keeps 1e-12.
+    tol = 1e-12 if not log_resp else 1e-9
+    assert max(errs.values()) <= tol, {k: f"{v:.2e}" for k, v in errs.items()}
     hK, hR = ctx.rhs()
     oK, oR = orc.rhs()
     rel_close(hK, oK, 1e-12)
@@ -123,10 +127,10 @@ def test_rk_step_teacher_forced(method):
             assert np.array_equal(b1, o1) and np.array_equal(b2, o2)
 
 
-@pytest.mark.parametrize("N,Tmax,steps", [(64, 1e2, 300), (128, 1e4, 150)])
-def test_quench_free_running(N, Tmax, steps):
+@pytest.mark.parametrize("N,Tmax,steps,log_resp", [(64, 1e2, 300, 0), (128, 1e4, 150, 0), (64, 1e2, 200, 1)])
+def test_quench_free_running(N, Tmax, steps, log_resp):
     g = grid(N, Tmax)
-    ctx, orc = pair(g, Gamma=0.8)
+    ctx, orc = pair(g, Gamma=0.8, log_resp=log_resp)
     ctx.init_quench()
     orc.init_quench()
     same_dt = 0
