@@ -41,7 +41,7 @@ class Grid(C.Structure):
 class Params(C.Structure):
     _fields_ = [("p", C.c_int32), ("p2", C.c_int32), ("lambda_", C.c_double), ("T0", C.c_double),
                 ("Gamma", C.c_double), ("delta_t_min", C.c_double), ("delta_max", C.c_double),
-                ("rmax", C.c_double * 11), ("log_response", C.c_int32), ("reserved", C.c_int32)]
+                ("rmax", C.c_double * 11), ("log_response", C.c_int32), ("maint_every", C.c_int32)]
 
 
 def _ptr(a):
@@ -300,3 +300,97 @@ def copy_probe(nbytes, reps=10):
     ms = C.c_double()
     _check(lib().dmfe_copy_probe(C.c_int64(int(nbytes)), C.c_int(reps), C.byref(ms)), "dmfe_copy_probe")
     return ms.value
+
+
+BATCH_RUNNING, BATCH_DONE = 0, 1
+
+
+class Batch:
+    """n independent quenches advanced together with device-side step control (dmfe_batch).
+
+    grids: one GridPackage per instance (instances given the same package share its
+    device tables); params: one Params per instance (same log_response)."""
+
+    def __init__(self, grids, params, rows_cap=256):
+        if len(grids) != len(params) or not grids:
+            raise ValueError("Batch: need one grid and one params per instance")
+        self.n = len(grids)
+        self.grids = list(grids)
+        self.N = [g.N for g in grids]
+        self._gs = (Grid * self.n)(*[g.as_struct() for g in grids])
+        self._ps = (Params * self.n)(*params)
+        self._h = C.c_void_p()
+        _check(lib().dmfe_batch_create(C.c_int(self.n), self._gs, self._ps, C.c_int64(rows_cap), C.byref(self._h)),
+               "dmfe_batch_create")
+
+    def close(self):
+        if self._h:
+            lib().dmfe_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init_quench(self):
+        _check(lib().dmfe_batch_init_quench(self._h), "dmfe_batch_init_quench")
+
+    def set_limits(self, tmax=float("inf"), max_loop=2**62):
+        _check(lib().dmfe_batch_set_limits(self._h, C.c_double(tmax), C.c_int64(max_loop)), "dmfe_batch_set_limits")
+
+    def step(self):
+        """One batch step (direct launches); returns the number of running instances."""
+        r = C.c_int()
+        _check(lib().dmfe_batch_step(self._h, C.byref(r)), "dmfe_batch_step")
+        return r.value
+
+    def run(self, max_steps, use_graph=True, chunk=16):
+        n = C.c_int64()
+        _check(lib().dmfe_batch_run(self._h, C.c_int64(max_steps), C.c_int(1 if use_graph else 0), C.c_int(chunk),
+                                    C.byref(n)), "dmfe_batch_run")
+        return n.value
+
+    def T1(self, i):
+        v = C.c_int64()
+        _check(lib().dmfe_batch_get_T1(self._h, C.c_int(i), C.byref(v)), "dmfe_batch_get_T1")
+        return v.value
+
+    def get_control(self, i):
+        dt, de, do, lp, me, st = C.c_double(), C.c_double(), C.c_double(), C.c_int64(), C.c_int(), C.c_int()
+        _check(lib().dmfe_batch_get_control(self._h, C.c_int(i), C.byref(dt), C.byref(de), C.byref(do), C.byref(lp),
+                                            C.byref(me), C.byref(st)), "dmfe_batch_get_control")
+        return dict(delta_t=dt.value, delta=de.value, delta_old=do.value, loop=lp.value, method=me.value,
+                    status=st.value)
+
+    def set_control(self, i, delta_t, delta, delta_old, loop, method):
+        _check(lib().dmfe_batch_set_control(self._h, C.c_int(i), C.c_double(delta_t), C.c_double(delta),
+                                            C.c_double(delta_old), C.c_int64(loop), C.c_int(method)),
+               "dmfe_batch_set_control")
+
+    def get_state(self, i):
+        T1, N = self.T1(i), self.N[i]
+        out = {k: np.zeros(T1) for k in ("t1grid", "rvec", "drvec", "dtratio")}
+        for k in ("QK", "QR", "dQK", "dQR"):
+            out[k] = np.zeros(T1 * N)
+        _check(lib().dmfe_batch_get_state(self._h, C.c_int(i), _ptr(out["t1grid"]), _ptr(out["QK"]), _ptr(out["QR"]),
+                                          _ptr(out["dQK"]), _ptr(out["dQR"]), _ptr(out["rvec"]), _ptr(out["drvec"]),
+                                          _ptr(out["dtratio"])), "dmfe_batch_get_state")
+        return out
+
+    def record(self, cap):
+        self._rec_cap = int(cap)
+        _check(lib().dmfe_batch_record(self._h, C.c_int(int(cap))), "dmfe_batch_record")
+
+    def records(self, i):
+        """(cap, 4) array of (t, E, QK(t,0), loop) rows; row (loop-1) % cap holds loop."""
+        out = np.zeros((self._rec_cap, 4))
+        _check(lib().dmfe_batch_records(self._h, C.c_int(i), _ptr(out)), "dmfe_batch_records")
+        return out

@@ -21,7 +21,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{os.p
 # packed FP32 instructions stay disabled in device code
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
-SOURCES = ["dmfe_kernels.hip", "dmfe_ctx.cpp", "dmfe_grid.cpp"]
+SOURCES = ["dmfe_kernels.hip", "dmfe_ctx.cpp", "dmfe_grid.cpp", "dmfe_batch.cpp"]
 
 
 def _run(cmd):

@@ -1,0 +1,716 @@
+// Batched quenches: many independent instances (own T, lambda, L, history, dt and
+// integrator) advanced together, one launch per stage for all of them.  The
+// per-instance scalar state of the reference's main loop (SimulationConfig /
+// RKData) lives on the device (struct Ctl) and is updated by the begin / end
+// kernels, so a batch step is a fixed launch sequence that is captured once into a
+// hipGraph and replayed; the host only looks at the instances every few steps, to
+// stop, to grow a history arena, or to run the every-1e5-loops maintenance
+// (sparsify + interpolate, simulation_runner.cpp:161-233) that stays on the host.
+#include "dmfe_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+using namespace dmfe;
+
+namespace {
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return set_error(DMFE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+struct GridDev {
+    const dmfe_grid* key = nullptr;
+    int N = 0, m = 0;
+    long long P = 0;
+    double *theta = nullptr, *integ = nullptr, *phi2 = nullptr;
+    int* inds[3] = {nullptr, nullptr, nullptr};
+    double* w[3] = {nullptr, nullptr, nullptr};
+    Tables tables() const {
+        Tables T{};
+        T.N = N; T.m = m; T.P = P;
+        T.theta = theta; T.integ = integ; T.phi2 = phi2;
+        for (int f = 0; f < 3; ++f) { T.inds[f] = inds[f]; T.w[f] = w[f]; }
+        return T;
+    }
+};
+
+bool same_grid(const dmfe_grid* a, const dmfe_grid* b) {
+    if (a->N != b->N || a->m != b->m || a->theta != b->theta || a->integ != b->integ || a->phi2 != b->phi2)
+        return false;
+    for (int f = 0; f < 3; ++f)
+        if (a->inds[f] != b->inds[f] || a->weights[f] != b->weights[f]) return false;
+    return true;
+}
+
+void free_hist(History& h) {
+    (void)hipFree(h.V); (void)hipFree(h.D); (void)hipFree(h.t1); (void)hipFree(h.rho); (void)hipFree(h.r);
+    (void)hipFree(h.dr);
+    h = History{};
+}
+
+int alloc_hist(History& h, int64_t rows, int N) {
+    const size_t n = (size_t)rows;
+    if (hipMalloc(&h.V, n * N * sizeof(double2)) != hipSuccess || hipMalloc(&h.D, n * N * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&h.t1, n * sizeof(double)) != hipSuccess || hipMalloc(&h.rho, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&h.r, n * sizeof(double)) != hipSuccess || hipMalloc(&h.dr, n * sizeof(double)) != hipSuccess) {
+        free_hist(h);
+        return set_error(DMFE_ERR_NOMEM, "batch: history allocation of " + std::to_string(rows) + " rows failed");
+    }
+    return DMFE_OK;
+}
+
+struct Inst {
+    int gi = 0;
+    dmfe_params prm{};
+    Model M{};
+    History H{};
+    Work W{};
+    int split = 1;
+    int64_t cap = 0;
+};
+
+}  // namespace
+
+struct dmfe_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<GridDev> grids;
+    std::vector<Inst> inst;
+    std::vector<Ctl> ctl;  // host copy, refreshed at synchronisation points
+    InstDesc* d_desc = nullptr;
+    Ctl* d_ctl = nullptr;
+    int2* d_smap = nullptr;
+    int4* d_rmap = nullptr;
+    int nb_stage = 0, nb_rhs = 0, ept = 1, G = 8, log_response = 0;
+    size_t lds_rhs = 0;
+    double* d_obs = nullptr;
+    int obs_cap = 0;
+    bool obs_on = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // 7- and 10-stage step sequences
+    int graph_steps[2] = {0, 0};
+    bool graph_obs[2] = {false, false};
+    int64_t steps_run = 0;
+};
+
+namespace {
+
+int n_inst(const dmfe_batch* b) { return (int)b->inst.size(); }
+
+int upload_desc(dmfe_batch* b) {
+    std::vector<InstDesc> d(b->inst.size());
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+        const Inst& x = b->inst[i];
+        d[i].T = b->grids[(size_t)x.gi].tables();
+        d[i].H = x.H;
+        d[i].W = x.W;
+        d[i].M = x.M;
+        d[i].ctl = b->d_ctl + i;
+        d[i].split = x.split;
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_desc, d.data(), d.size() * sizeof(InstDesc), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return DMFE_OK;
+}
+
+int pull_ctl(dmfe_batch* b) {
+    HIP_TRY(hipMemcpyAsync(b->ctl.data(), b->d_ctl, b->ctl.size() * sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return DMFE_OK;
+}
+
+int push_ctl(dmfe_batch* b) {
+    HIP_TRY(hipMemcpyAsync(b->d_ctl, b->ctl.data(), b->ctl.size() * sizeof(Ctl), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return DMFE_OK;
+}
+
+void drop_graphs(dmfe_batch* b) {
+    for (int k = 0; k < 2; ++k) {
+        if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
+        b->graph[k] = nullptr;
+        b->graph_steps[k] = 0;
+    }
+}
+
+// single-instance interpolation (search B) of instance i, outside a step
+int interpolate_inst(dmfe_batch* b, int i) {
+    Inst& x = b->inst[(size_t)i];
+    Ctl& c = b->ctl[(size_t)i];
+    double tl = 0.0;
+    HIP_TRY(hipMemcpyAsync(&tl, x.H.t1 + (c.T1 - 1), sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    SearchArgs a{};
+    a.mode = kSearchB;
+    a.T1 = c.T1;
+    a.t1 = x.H.t1;
+    a.tl = tl;
+    a.fbuf = 1 - c.fcur;
+    launch_stage(b->grids[(size_t)x.gi].tables(), x.H, x.W, x.M, &a, nullptr, b->stream);
+    HIP_TRY(hipGetLastError());
+    c.fcur = a.fbuf;
+    return DMFE_OK;
+}
+
+// One batch step: begin | (stage n, rhs n) for n < S | stage S | end [| observables]
+int enqueue_step(dmfe_batch* b, int S) {
+    const int n = n_inst(b);
+    launch_batch_begin(b->d_desc, n, b->stream);
+    for (int k = 0; k < S; ++k) {
+        launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);
+        launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->lds_rhs, b->stream);
+    }
+    launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, S, b->ept, b->stream);
+    launch_batch_end(b->d_desc, n, b->stream);
+    if (b->obs_on) launch_batch_obs(b->d_desc, n, b->d_obs, b->obs_cap, b->stream);
+    HIP_TRY(hipGetLastError());
+    return DMFE_OK;
+}
+
+int max_stages(const dmfe_batch* b) {
+    int S = 0;
+    for (const Ctl& c : b->ctl)
+        if (c.status == kRunning) S = std::max(S, c.method == DMFE_RK54 ? 7 : 10);
+    return S;
+}
+
+int build_graph(dmfe_batch* b, int which, int S, int steps) {
+    if (b->graph[which] && b->graph_steps[which] == steps && b->graph_obs[which] == b->obs_on) return DMFE_OK;
+    if (b->graph[which]) (void)hipGraphExecDestroy(b->graph[which]);
+    b->graph[which] = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DMFE_OK;
+    for (int s = 0; s < steps && rc == DMFE_OK; ++s) rc = enqueue_step(b, S);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph[which], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    b->graph_steps[which] = steps;
+    b->graph_obs[which] = b->obs_on;
+    return DMFE_OK;
+}
+
+// sparsifyNscale (CPU-only build semantics) on instance i; keeps the arena capacity
+int sparsify_inst(dmfe_batch* b, int i, double threshold) {
+    Inst& x = b->inst[(size_t)i];
+    Ctl& c = b->ctl[(size_t)i];
+    const int64_t T1 = c.T1;
+    const GridDev& gd = b->grids[(size_t)x.gi];
+    std::vector<int> keep{0};
+    bool erased = false;
+    if (T1 >= 4) {
+        double* dval = nullptr;
+        HIP_TRY(hipMalloc(&dval, (size_t)T1 * sizeof(double)));
+        launch_sparsify_values(gd.tables(), x.H, (int)T1, dval, b->stream);
+        std::vector<double> val((size_t)T1, 0.0);
+        hipError_t e = hipMemcpyAsync(val.data(), dval, (size_t)T1 * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        (void)hipFree(dval);
+        if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("batch sparsify: ") + hipGetErrorString(e));
+        for (int64_t r = 2; r + 1 < T1; ++r) {
+            if (val[(size_t)r] < threshold) erased = true;
+            else keep.push_back((int)r);
+        }
+    }
+    keep.push_back((int)(T1 - 1));
+    if (!erased) return DMFE_OK;
+    History h{};
+    int rc = alloc_hist(h, x.cap, gd.N);
+    if (rc) return rc;
+    int* dkeep = nullptr;
+    HIP_TRY(hipMalloc(&dkeep, keep.size() * sizeof(int)));
+    HIP_TRY(hipMemcpyAsync(dkeep, keep.data(), keep.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    launch_gather_rows(gd.tables(), x.H, h, dkeep, (int)keep.size(), b->stream);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    (void)hipFree(dkeep);
+    free_hist(x.H);
+    x.H = h;
+    c.T1 = (int)keep.size();
+    return DMFE_OK;
+}
+
+// the rest of the main-loop body after loop % cadence == 0 (simulation_runner.cpp:161-278)
+int maintenance(dmfe_batch* b, int i) {
+    Inst& x = b->inst[(size_t)i];
+    Ctl& c = b->ctl[(size_t)i];
+    const double dmax = x.prm.delta_max;
+    int64_t prev = c.T1;
+    for (int count = 0; count < 1; ++count) {
+        int rc = sparsify_inst(b, i, dmax);
+        if (rc) return rc;
+        if (c.T1 >= prev) break;
+        prev = c.T1;
+    }
+    int rc = interpolate_inst(b, i);
+    if (rc) return rc;
+    if (c.delta < dmax / 2 && c.loop - (-1000) > 1000) {
+        c.dt *= 0.5;
+        c.method = DMFE_SSPRK104;
+    }
+    double rho_last = 0.0;
+    HIP_TRY(hipMemcpyAsync(&rho_last, x.H.rho + (c.T1 - 1), sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    const double* rmax = x.prm.rmax;
+    if (c.delta < dmax && c.loop > 5 && (c.delta < 1.1 * c.delta_old || c.delta_old < dmax / 1000) &&
+        rmax[c.method - 1] / c.specRad > c.dt && rho_last == 1.0) {
+        c.dt *= 1.01;
+    } else if (c.delta > 2 * dmax && c.dt > x.prm.delta_t_min) {
+        c.dt *= 0.9;
+    }
+    if (c.method == DMFE_SSPRK104 && c.delta > dmax && rmax[0] / c.specRad > c.dt) {
+        c.method = DMFE_RK54;
+        c.dt *= 0.5;
+    }
+    c.status = kRunning;
+    return DMFE_OK;
+}
+
+int grow_inst(dmfe_batch* b, int i) {
+    Inst& x = b->inst[(size_t)i];
+    Ctl& c = b->ctl[(size_t)i];
+    const int N = b->grids[(size_t)x.gi].N;
+    const int64_t nc = std::max<int64_t>(2 * x.cap, c.T1 + 64);
+    History h{};
+    int rc = alloc_hist(h, nc, N);
+    if (rc) return rc;
+    const size_t n = (size_t)c.T1;
+    HIP_TRY(hipMemcpyAsync(h.V, x.H.V, n * N * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(h.D, x.H.D, n * N * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(h.t1, x.H.t1, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(h.rho, x.H.rho, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(h.r, x.H.r, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(h.dr, x.H.dr, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    free_hist(x.H);
+    x.H = h;
+    x.cap = nc;
+    c.cap = (int)nc;
+    c.status = kRunning;
+    return DMFE_OK;
+}
+
+// host-side service of the instances that stopped for the host
+int service(dmfe_batch* b, bool* changed) {
+    *changed = false;
+    for (int i = 0; i < n_inst(b); ++i) {
+        Ctl& c = b->ctl[(size_t)i];
+        int rc = DMFE_OK;
+        if (c.status == kNeedMaint) rc = maintenance(b, i);
+        else if (c.status == kNeedRows) rc = grow_inst(b, i);
+        else continue;
+        if (rc) return rc;
+        *changed = true;
+    }
+    if (*changed) {
+        int rc = upload_desc(b);
+        if (rc) return rc;
+        rc = push_ctl(b);
+        if (rc) return rc;
+        // captured graphs stay valid: kernels read the per-instance pointers from d_desc
+    }
+    return DMFE_OK;
+}
+
+void free_batch(dmfe_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (GridDev& g : b->grids) {
+        (void)hipFree(g.theta); (void)hipFree(g.integ); (void)hipFree(g.phi2);
+        for (int f = 0; f < 3; ++f) { (void)hipFree(g.inds[f]); (void)hipFree(g.w[f]); }
+    }
+    for (Inst& x : b->inst) {
+        free_hist(x.H);
+        Work& W = x.W;
+        for (void* p : {(void*)W.posB1, (void*)W.posB2, (void*)W.saveB1, (void*)W.saveB2, (void*)W.Floor, (void*)W.diff,
+                        (void*)W.rInt, (void*)W.partial, (void*)W.qkb1c0, (void*)W.hK, (void*)W.hR, (void*)W.scal})
+            (void)hipFree(p);
+    }
+    (void)hipFree(b->d_desc); (void)hipFree(b->d_ctl); (void)hipFree(b->d_smap); (void)hipFree(b->d_rmap);
+    (void)hipFree(b->d_obs);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int upload_grid(GridDev& gd, const dmfe_grid* g) {
+    const int N = (int)g->N, m = (int)g->m;
+    const long long P = (long long)N * N;
+    gd.key = g; gd.N = N; gd.m = m; gd.P = P;
+    const size_t bN = (size_t)N * sizeof(double), bP = (size_t)P * sizeof(double);
+    HIP_TRY(hipMalloc(&gd.theta, bN));
+    HIP_TRY(hipMalloc(&gd.integ, bN));
+    HIP_TRY(hipMalloc(&gd.phi2, bP));
+    HIP_TRY(hipMemcpy(gd.theta, g->theta, bN, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(gd.integ, g->integ, bN, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(gd.phi2, g->phi2, bP, hipMemcpyHostToDevice));
+    std::vector<double> wt((size_t)P * m);
+    std::vector<int> it((size_t)P);
+    for (int f = 0; f < 3; ++f) {
+        for (long long k = 0; k < P; ++k) {
+            it[(size_t)k] = g->inds[f][k] < 0 ? 0 : (int)g->inds[f][k];
+            if (f < 2)
+                for (int d = 0; d < m; ++d) wt[(size_t)d * P + k] = g->weights[f][(size_t)k * m + d];
+        }
+        HIP_TRY(hipMalloc(&gd.inds[f], (size_t)P * sizeof(int)));
+        HIP_TRY(hipMalloc(&gd.w[f], bP * (size_t)m));
+        HIP_TRY(hipMemcpy(gd.inds[f], it.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gd.w[f], f < 2 ? wt.data() : g->weights[f], bP * (size_t)m, hipMemcpyHostToDevice));
+    }
+    return DMFE_OK;
+}
+
+int alloc_work(Work& W, int N, int split) {
+    const size_t bN = (size_t)N * sizeof(double), bP = (size_t)N * N * sizeof(double);
+    HIP_TRY(hipMalloc(&W.posB1, bN));
+    HIP_TRY(hipMalloc(&W.posB2, bP));
+    HIP_TRY(hipMalloc(&W.saveB1, 3 * bN));
+    HIP_TRY(hipMalloc(&W.saveB2, 3 * bP));
+    HIP_TRY(hipMalloc(&W.Floor, 2 * (size_t)N * sizeof(int)));
+    HIP_TRY(hipMalloc(&W.diff, 2 * bN));
+    HIP_TRY(hipMalloc(&W.rInt, bN));
+    HIP_TRY(hipMalloc(&W.partial, bN * (size_t)split * kSums));
+    HIP_TRY(hipMalloc(&W.qkb1c0, bN));
+    HIP_TRY(hipMalloc(&W.hK, bN * kMaxStages));
+    HIP_TRY(hipMalloc(&W.hR, bN * kMaxStages));
+    HIP_TRY(hipMalloc(&W.scal, 8 * sizeof(double)));
+    HIP_TRY(hipMemset(W.scal, 0, 8 * sizeof(double)));
+    return DMFE_OK;
+}
+
+}  // namespace
+
+// ============================== C ABI ==============================================
+
+extern "C" int dmfe_batch_create(int n, const dmfe_grid* grids, const dmfe_params* params, int64_t rows_cap,
+                                 dmfe_batch** out) {
+    if (!grids || !params || !out || n < 1) return set_error(DMFE_ERR_ARG, "dmfe_batch_create: bad argument");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        const dmfe_grid* g = grids + i;
+        if (g->N < 4 || g->N > kMaxN || g->m < 1 || g->m > g->N || !g->theta || !g->integ || !g->phi2)
+            return set_error(DMFE_ERR_ARG, "dmfe_batch_create: bad grid for instance " + std::to_string(i));
+        for (int f = 0; f < 3; ++f)
+            if (!g->inds[f] || !g->weights[f]) return set_error(DMFE_ERR_ARG, "dmfe_batch_create: missing stencil table");
+        if (params[i].log_response != params[0].log_response)
+            return set_error(DMFE_ERR_ARG, "dmfe_batch_create: log_response must be the same for all instances");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(DMFE_ERR_HIP, "dmfe_batch_create: no HIP device");
+    dmfe_batch* b = new dmfe_batch();
+    (void)hipGetDevice(&b->device);
+    auto fail = [&](int rc) { std::string msg = dmfe_last_error(); free_batch(b); return set_error(rc, msg); };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_error(DMFE_ERR_HIP, "dmfe_batch_create: stream"));
+    batch_upload_tableaux(dp54(), ssprk104());
+    b->log_response = params[0].log_response;
+    b->inst.resize((size_t)n);
+    b->ctl.assign((size_t)n, Ctl{});
+    const int64_t cap = std::max<int64_t>(rows_cap, 64);
+    int maxm = 1;
+    for (int i = 0; i < n; ++i) {
+        const dmfe_grid* g = grids + i;
+        int gi = -1;
+        for (size_t k = 0; k < b->grids.size(); ++k)
+            if (same_grid(b->grids[k].key, g)) gi = (int)k;
+        if (gi < 0) {
+            for (int f = 0; f < 3; ++f)
+                for (long long k = 0; k < g->N * g->N; ++k) {
+                    const int64_t s = g->inds[f][k] < 0 ? 0 : g->inds[f][k];
+                    if (s + g->m > g->N) return fail(set_error(DMFE_ERR_ARG, "dmfe_batch_create: stencil exceeds N"));
+                }
+            b->grids.emplace_back();
+            int rc = upload_grid(b->grids.back(), g);
+            if (rc) return fail(rc);
+            gi = (int)b->grids.size() - 1;
+        }
+        Inst& x = b->inst[(size_t)i];
+        x.gi = gi;
+        x.prm = params[i];
+        x.M.lambda = params[i].lambda; x.M.T0 = params[i].T0; x.M.Gamma = params[i].Gamma;
+        x.M.p = params[i].p; x.M.p2 = params[i].p2;
+        const int N = (int)g->N;
+        x.split = (N >= 256) ? std::max(1, N / 256) : 1;
+        x.cap = cap;
+        int rc = alloc_hist(x.H, cap, N);
+        if (rc) return fail(rc);
+        rc = alloc_work(x.W, N, x.split);
+        if (rc) return fail(rc);
+        maxm = std::max(maxm, (int)g->m);
+        const int cw = (N + x.split - 1) / x.split;
+        b->lds_rhs = std::max(b->lds_rhs, (size_t)(N + cw) * sizeof(double2) + (size_t)cw * (sizeof(double) + 2 * sizeof(int)));
+        b->ept = std::max(b->ept, (N + 255) / 256);
+    }
+    b->G = maxm <= 16 ? 8 : (maxm <= 32 ? 16 : 64);
+    // stage map: finish blocks, theta blocks, then the phi2 rows of every instance
+    std::vector<int2> smap;
+    for (int i = 0; i < n; ++i) smap.push_back(make_int2(i, -2));
+    for (int i = 0; i < n; ++i) smap.push_back(make_int2(i, -1));
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < b->grids[(size_t)b->inst[(size_t)i].gi].N; ++k) smap.push_back(make_int2(i, k));
+    // rhs map: for each grid, rows x chunks x the instances sharing that grid
+    std::vector<int4> rmap;
+    for (size_t gi = 0; gi < b->grids.size(); ++gi) {
+        std::vector<int> members;
+        for (int i = 0; i < n; ++i)
+            if (b->inst[(size_t)i].gi == (int)gi) members.push_back(i);
+        const int N = b->grids[gi].N;
+        const int split = b->inst[(size_t)members[0]].split;
+        for (int j = 0; j < N; ++j)
+            for (int ch = 0; ch < split; ++ch)
+                for (int i : members) rmap.push_back(make_int4(i, j, ch, 0));
+    }
+    while (rmap.size() % 8) rmap.push_back(make_int4(-1, 0, 0, 0));
+    b->nb_stage = (int)smap.size();
+    b->nb_rhs = (int)rmap.size();
+    if (hipMalloc(&b->d_desc, (size_t)n * sizeof(InstDesc)) != hipSuccess ||
+        hipMalloc(&b->d_ctl, (size_t)n * sizeof(Ctl)) != hipSuccess ||
+        hipMalloc(&b->d_smap, smap.size() * sizeof(int2)) != hipSuccess ||
+        hipMalloc(&b->d_rmap, rmap.size() * sizeof(int4)) != hipSuccess)
+        return fail(set_error(DMFE_ERR_NOMEM, "dmfe_batch_create: allocation failed"));
+    if (hipMemcpy(b->d_smap, smap.data(), smap.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_rmap, rmap.data(), rmap.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_error(DMFE_ERR_HIP, "dmfe_batch_create: map upload failed"));
+    for (int i = 0; i < n; ++i) {
+        Ctl& c = b->ctl[(size_t)i];
+        const dmfe_params& p = params[i];
+        c.status = kDone;  // until dmfe_batch_init_quench
+        c.cap = (int)cap;
+        c.dt_min = p.delta_t_min;
+        c.delta_max = p.delta_max;
+        c.specRad = 4 * std::sqrt(host_fl(p, 1, 2));
+        std::memcpy(c.rmax, p.rmax, sizeof c.rmax);
+        c.tmax = std::numeric_limits<double>::infinity();
+        c.max_loop = std::numeric_limits<long long>::max();
+        c.maint_every = p.maint_every > 0 ? p.maint_every : 100000;
+    }
+    int rc = upload_desc(b);
+    if (rc) return fail(rc);
+    rc = push_ctl(b);
+    if (rc) return fail(rc);
+    *out = b;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_destroy(dmfe_batch* b) {
+    free_batch(b);
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_size(const dmfe_batch* b, int* n) {
+    if (!b || !n) return set_error(DMFE_ERR_ARG, "null argument");
+    *n = n_inst(b);
+    return DMFE_OK;
+}
+
+// initialization.cpp:204-223, 277-283 for every instance
+extern "C" int dmfe_batch_init_quench(dmfe_batch* b) {
+    if (!b) return set_error(DMFE_ERR_ARG, "null batch");
+    drop_graphs(b);
+    for (int i = 0; i < n_inst(b); ++i) {
+        Inst& x = b->inst[(size_t)i];
+        Ctl& c = b->ctl[(size_t)i];
+        const GridDev& gd = b->grids[(size_t)x.gi];
+        const int N = gd.N;
+        std::vector<double2> v((size_t)N, make_double2(1.0, 1.0)), d((size_t)N, make_double2(0.0, 0.0));
+        const double zero = 0.0;
+        const double r0 = x.prm.Gamma + host_fl(x.prm, 1, 1) / x.prm.T0;
+        HIP_TRY(hipMemcpyAsync(x.H.V, v.data(), N * sizeof(double2), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(x.H.D, d.data(), N * sizeof(double2), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(x.H.t1, &zero, sizeof(double), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(x.H.rho, &zero, sizeof(double), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(x.H.r, &r0, sizeof(double), hipMemcpyHostToDevice, b->stream));
+        launch_row_scalar(gd.tables(), x.H, x.W, x.M, 0, 0.0, 0, b->stream);
+        HIP_TRY(hipMemcpyAsync(x.H.dr, x.W.scal + 2, sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        c.T1 = 1;
+        c.dt = x.prm.delta_t_min;
+        c.loop = 0;
+        c.delta = 1;
+        c.delta_old = 0;
+        c.t_prop = 0.0;
+        c.fcur = 0;
+        c.method = (c.dt < x.prm.rmax[0] / c.specRad) ? DMFE_RK54 : DMFE_SSPRK104;
+        c.status = kRunning;
+        c.active = 0;
+        // first interpolation (initialization.cpp:283)
+        int rc = interpolate_inst(b, i);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->steps_run = 0;
+    return push_ctl(b);
+}
+
+// One batch step for every running instance (direct launches, then a synchronisation).
+extern "C" int dmfe_batch_step(dmfe_batch* b, int* running) {
+    if (!b) return set_error(DMFE_ERR_ARG, "null batch");
+    const int S = max_stages(b);
+    if (S > 0) {
+        int rc = enqueue_step(b, S);
+        if (rc) return rc;
+        ++b->steps_run;
+    }
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    bool changed = false;
+    rc = service(b, &changed);
+    if (rc) return rc;
+    int nr = 0;
+    for (const Ctl& c : b->ctl) nr += c.status == kRunning;
+    if (running) *running = nr;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_set_limits(dmfe_batch* b, double tmax, int64_t max_loop) {
+    if (!b) return set_error(DMFE_ERR_ARG, "null batch");
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    for (Ctl& c : b->ctl) {
+        c.tmax = tmax;
+        c.max_loop = max_loop;
+        if (c.status == kDone && c.T1 >= 1) {
+            // re-test the loop condition at the next begin
+            const double eps = std::numeric_limits<double>::epsilon();
+            if (c.t_prop < tmax && c.loop < max_loop && c.dt >= c.dt_min - eps * std::max(std::fabs(c.dt), std::fabs(c.dt_min)))
+                c.status = kRunning;
+        }
+    }
+    return push_ctl(b);
+}
+
+// Runs batch steps until every instance stops (simulation_runner.cpp:125 per instance) or
+// max_steps batch steps ran.  use_graph: replay a captured hipGraph of `chunk` steps.
+extern "C" int dmfe_batch_run(dmfe_batch* b, int64_t max_steps, int use_graph, int chunk, int64_t* steps_out) {
+    if (!b || max_steps < 0) return set_error(DMFE_ERR_ARG, "dmfe_batch_run: bad argument");
+    chunk = std::max(1, chunk);
+    int64_t done = 0;
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    while (done < max_steps) {
+        const int S = max_stages(b);
+        if (S == 0) break;
+        const int steps = (int)std::min<int64_t>(chunk, max_steps - done);
+        if (use_graph && steps == chunk) {
+            const int which = S == 7 ? 0 : 1;
+            rc = build_graph(b, which, S, steps);
+            if (rc) return rc;
+            HIP_TRY(hipGraphLaunch(b->graph[which], b->stream));
+        } else {
+            for (int s = 0; s < steps && rc == DMFE_OK; ++s) rc = enqueue_step(b, S);
+            if (rc) return rc;
+        }
+        done += steps;
+        rc = pull_ctl(b);
+        if (rc) return rc;
+        bool changed = false;
+        rc = service(b, &changed);
+        if (rc) return rc;
+    }
+    b->steps_run += done;
+    if (steps_out) *steps_out = done;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_get_control(dmfe_batch* b, int i, double* delta_t, double* delta, double* delta_old,
+                                      int64_t* loop, int* method, int* status) {
+    if (!b || i < 0 || i >= n_inst(b)) return set_error(DMFE_ERR_ARG, "dmfe_batch_get_control: bad argument");
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    const Ctl& c = b->ctl[(size_t)i];
+    if (delta_t) *delta_t = c.dt;
+    if (delta) *delta = c.delta;
+    if (delta_old) *delta_old = c.delta_old;
+    if (loop) *loop = c.loop;
+    if (method) *method = c.method;
+    if (status) *status = c.status;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_set_control(dmfe_batch* b, int i, double delta_t, double delta, double delta_old, int64_t loop,
+                                      int method) {
+    if (!b || i < 0 || i >= n_inst(b) || (method != DMFE_RK54 && method != DMFE_SSPRK104))
+        return set_error(DMFE_ERR_ARG, "dmfe_batch_set_control: bad argument");
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    Ctl& c = b->ctl[(size_t)i];
+    if (c.T1 < 1) return set_error(DMFE_ERR_STATE, "dmfe_batch_set_control: instance has no state");
+    c.dt = delta_t;
+    c.delta = delta;
+    c.delta_old = delta_old;
+    c.loop = loop;
+    c.method = method;
+    return push_ctl(b);
+}
+
+extern "C" int dmfe_batch_get_T1(dmfe_batch* b, int i, int64_t* T1) {
+    if (!b || !T1 || i < 0 || i >= n_inst(b)) return set_error(DMFE_ERR_ARG, "dmfe_batch_get_T1: bad argument");
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    *T1 = b->ctl[(size_t)i].T1;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_get_state(dmfe_batch* b, int i, double* t1grid, double* QK, double* QR, double* dQK,
+                                    double* dQR, double* rvec, double* drvec, double* dtratio) {
+    if (!b || i < 0 || i >= n_inst(b)) return set_error(DMFE_ERR_ARG, "dmfe_batch_get_state: bad argument");
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    const Inst& x = b->inst[(size_t)i];
+    const size_t N = (size_t)b->grids[(size_t)x.gi].N, n = (size_t)b->ctl[(size_t)i].T1;
+    auto get = [&](double* dst, const double* src) -> int {
+        if (!dst) return DMFE_OK;
+        HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        return DMFE_OK;
+    };
+    if ((rc = get(t1grid, x.H.t1)) || (rc = get(rvec, x.H.r)) || (rc = get(drvec, x.H.dr)) || (rc = get(dtratio, x.H.rho)))
+        return rc;
+    if (QK || QR || dQK || dQR) {
+        std::vector<double2> v(n * N), d(n * N);
+        HIP_TRY(hipMemcpyAsync(v.data(), x.H.V, n * N * sizeof(double2), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipMemcpyAsync(d.data(), x.H.D, n * N * sizeof(double2), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        for (size_t k = 0; k < n * N; ++k) {
+            if (QK) QK[k] = v[k].x;
+            if (QR) QR[k] = v[k].y;
+            if (dQK) dQK[k] = d[k].x;
+            if (dQR) dQR[k] = d[k].y;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return DMFE_OK;
+}
+
+// Per-step correlation records (t, E(t), QK(t,0), loop) of every instance, kept in a
+// device ring of `cap` rows per instance; cap <= 0 switches recording off.
+extern "C" int dmfe_batch_record(dmfe_batch* b, int cap) {
+    if (!b) return set_error(DMFE_ERR_ARG, "null batch");
+    (void)hipFree(b->d_obs);
+    b->d_obs = nullptr;
+    b->obs_cap = 0;
+    b->obs_on = false;
+    drop_graphs(b);
+    if (cap <= 0) return DMFE_OK;
+    HIP_TRY(hipMalloc(&b->d_obs, (size_t)n_inst(b) * cap * 4 * sizeof(double)));
+    HIP_TRY(hipMemset(b->d_obs, 0, (size_t)n_inst(b) * cap * 4 * sizeof(double)));
+    b->obs_cap = cap;
+    b->obs_on = true;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_records(dmfe_batch* b, int i, double* out) {
+    if (!b || !out || i < 0 || i >= n_inst(b) || !b->obs_on) return set_error(DMFE_ERR_ARG, "dmfe_batch_records: bad argument");
+    HIP_TRY(hipMemcpyAsync(out, b->d_obs + (size_t)i * b->obs_cap * 4, (size_t)b->obs_cap * 4 * sizeof(double),
+                           hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return DMFE_OK;
+}

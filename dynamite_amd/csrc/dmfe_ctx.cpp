@@ -25,8 +25,6 @@ int set_error(int code, const std::string& msg) {
     return code;
 }
 
-namespace {
-
 #define HIP_TRY(expr)                                                                                     \
     do {                                                                                                  \
         hipError_t e_ = (expr);                                                                           \
@@ -35,13 +33,6 @@ namespace {
     } while (0)
 
 // Dormand-Prince 5(4) and SSPRK(10,4) tableaux (runge_kutta.cpp:22-35,149-159)
-struct Tableau {
-    int S;
-    double A[kMaxStages][kMaxStages];
-    double b[kMaxStages];
-    double b2[kMaxStages];
-};
-
 const Tableau& dp54() {
     static Tableau t = [] {
         Tableau x{};
@@ -100,7 +91,6 @@ double host_fl(const dmfe_params& p, double q, int d) {
     }
 }
 
-}  // namespace
 }  // namespace dmfe
 
 using namespace dmfe;
@@ -429,7 +419,8 @@ int loop_iteration(dmfe_ctx* c, double* t_out) {
     c->delta = err;
     c->loop++;
     const double dmax = c->params.delta_max;
-    if (c->loop % 100000 == 0) {
+    const int64_t every = c->params.maint_every > 0 ? c->params.maint_every : 100000;
+    if (c->loop % every == 0) {
         int64_t prev = c->T1;
         int count = 0;
         while (count < 1) {
@@ -507,6 +498,7 @@ extern "C" void dmfe_params_default(dmfe_params* p) {
     const double r[11] = {3, 13, 20, 80, 180, 320, 500, 720, 980, 1280, 1620};
     std::memcpy(p->rmax, r, sizeof r);
     p->log_response = 0;
+    p->maint_every = 100000;
 }
 
 extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64_t rows_hint, dmfe_ctx** out) {

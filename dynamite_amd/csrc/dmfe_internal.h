@@ -125,4 +125,61 @@ void launch_gather_rows(const Tables& T, const History& src, const History& dst,
 
 void launch_hbm_copy(const double2* src, double2* dst, long long n, hipStream_t s);
 
+// ---- batched quenches with device-side step control ----------------------------------
+enum CtlStatus { kRunning = 0, kDone = 1, kNeedMaint = 2, kNeedRows = 3 };
+
+// Per-instance control state (the SimulationConfig / RKData scalars of one quench),
+// owned by the device while a batch runs.
+struct Ctl {
+    // persistent
+    int T1;             // history rows
+    int method;         // DMFE_RK54 / DMFE_SSPRK104
+    int status;         // CtlStatus
+    int fcur;           // Floor/diff buffer of the latest interpolation
+    long long loop;
+    double dt, delta, delta_old;
+    double t_prop;      // proposed t of the last iteration (simulation_runner.cpp:130)
+    // limits
+    double tmax;
+    long long max_loop;
+    long long maint_every;
+    int cap;            // history capacity (rows)
+    int active;         // takes part in the current step
+    double dt_min, delta_max, specRad;
+    double rmax[11];
+    // per-step scratch, set by the begin kernel
+    int T1s, S, fc0, pad;
+    double gt[kMaxStages + 1];
+    double tprev;       // t1grid[T1s - 2]
+    double gt_err;      // |gt_S - gte|
+};
+
+struct InstDesc {
+    Tables T;
+    History H;
+    Work W;
+    Model M;
+    Ctl* ctl;
+    int split;
+    int pad;
+};
+
+// Butcher tableaux of the two integrators and f^(d)(q) on the host (dmfe_ctx.cpp)
+struct Tableau {
+    int S;
+    double A[kMaxStages][kMaxStages];
+    double b[kMaxStages];
+    double b2[kMaxStages];
+};
+const Tableau& dp54();
+const Tableau& ssprk104();
+double host_fl(const dmfe_params& p, double q, int d);
+void batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk);
+void launch_batch_begin(InstDesc* D, int n, hipStream_t s);
+void launch_batch_end(InstDesc* D, int n, hipStream_t s);
+void launch_batch_stage(InstDesc* D, const int2* map, int nblocks, int n, int ept, hipStream_t s);
+void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_response, int G, size_t lds,
+                      hipStream_t s);
+void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t s);
+
 }  // namespace dmfe

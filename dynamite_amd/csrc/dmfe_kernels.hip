@@ -376,15 +376,11 @@ __device__ inline double2 stencil_last(const double* __restrict__ w, long long P
 //          Hermite (index_vec.cpp:63-136), the self-energies and the weighted
 //          products, then one block reduction.
 template <bool LOGR, int G>
-__global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History H, Work W, Model M, RhsArgs a) {
-    extern __shared__ double2 dyn[];  // [N] last history row, [cw] (QKB2, QRB2), [cw] u, [cw] ix, s
-    __shared__ double red[kSums * (kRowThreads / kWave)];
+__device__ void rhs_rows_body(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
+                              int j, int chunk, double2* dyn /* [N] last row, [cw] (QKB2, QRB2), [cw] u, [cw] ix, s */,
+                              double* red) {
     const int N = T.N, m = T.m;
     const long long P = T.P;
-    const int nb = gridDim.x;
-    const int lb = xcd_remap(blockIdx.x, nb);
-    const int j = lb / a.split;
-    const int chunk = lb % a.split;
     const int cw = (N + a.split - 1) / a.split;
     const int c0 = chunk * cw, c1 = min(N, c0 + cw);
     const int T1 = a.T1;
@@ -549,6 +545,14 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History
     if (threadIdx.x < kSums) W.partial[((size_t)j * a.split + chunk) * kSums + threadIdx.x] = acc[threadIdx.x];
 }
 
+template <bool LOGR, int G>
+__global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History H, Work W, Model M, RhsArgs a) {
+    extern __shared__ double2 dyn[];
+    __shared__ double red[kSums * (kRowThreads / kWave)];
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    rhs_rows_body<LOGR, G>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red);
+}
+
 // ---- finish: RHS epilogue + RK stage -------------------------------------------------
 template <int EPT>
 __device__ void finish_block(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
@@ -708,9 +712,10 @@ __global__ __launch_bounds__(kStageThreads) void stage_kernel(Tables T, History 
 }
 
 // rstep (what 0) / energy (what 1) on one history row
-__global__ __launch_bounds__(kStageThreads) void row_scalar_kernel(Tables T, History H, Work W, Model M, int row,
-                                                                   double t, int what) {
-    __shared__ double red[2 * (kStageThreads / kWave)];
+// rstep (what 0, time_steps.cpp:83-92) or the energy (what 1, simulation_runner.cpp:308-318)
+// of history row `row` at time t; the value is returned to thread 0.
+__device__ double row_scalar_body(const Tables& T, const History& H, const Model& M, int row, double t, int what,
+                                  double* red /* 2 * kStageThreads / 64 */) {
     const int N = T.N;
     const double2* v = H.V + (size_t)row * N;
     double s[2] = {0.0, 0.0};
@@ -725,11 +730,16 @@ __global__ __launch_bounds__(kStageThreads) void row_scalar_kernel(Tables T, His
         }
     }
     block_sum<2, kStageThreads>(s, red);
-    if (threadIdx.x == 0) {
-        const double qK0 = v[0].x;
-        if (what == 0) W.scal[2] = M.Gamma + s[0] + s[1] + fl1(M, qK0) * qK0 / M.T0;
-        else W.scal[2] = -(s[0] + fl1(M, qK0) / M.T0);
-    }
+    const double qK0 = v[0].x;
+    if (what == 0) return M.Gamma + s[0] + s[1] + fl1(M, qK0) * qK0 / M.T0;
+    return -(s[0] + fl1(M, qK0) / M.T0);
+}
+
+__global__ __launch_bounds__(kStageThreads) void row_scalar_kernel(Tables T, History H, Work W, Model M, int row,
+                                                                   double t, int what) {
+    __shared__ double red[2 * (kStageThreads / kWave)];
+    const double v = row_scalar_body(T, H, M, row, t, what, red);
+    if (threadIdx.x == 0) W.scal[2] = v;
 }
 
 // sparsifyNscale values (sparsify_utils.cpp:19-49): one workgroup per interior row i
@@ -769,6 +779,217 @@ __global__ __launch_bounds__(kRowThreads) void gather_rows_kernel(Tables T, Hist
         dst.rho[r] = src.rho[s];
         dst.r[r] = src.r[s];
         dst.dr[r] = src.dr[s];
+    }
+}
+
+// ---- batched quenches: device-side step control -------------------------------------
+// One batch step advances every running instance by one iteration of the main loop
+// (simulation_runner.cpp:125-278): begin (stage times) -> for each stage n the stage
+// launch (finish of n-1 + searches of n) and the RHS launch -> final stage launch ->
+// end (error norm -> dt rule).  Launch arguments are fixed; everything that varies is
+// read from the per-instance Ctl, so a step can be captured once into a hipGraph.
+// The scalar arithmetic repeats the reference's FP64 order without contraction.
+
+// [0] Dormand-Prince 5(4) (runge_kutta.cpp:149-159), [1] SSPRK(10,4) (:22-35)
+__constant__ double cA[2][kMaxStages][kMaxStages];
+__constant__ double cB[2][kMaxStages];
+__constant__ double cB2[2][kMaxStages];
+
+#pragma clang fp contract(off)
+__device__ inline int searches_before(const Ctl& c, int k) {  // searches in stages 0..k-1 of this step
+    return k <= 0 ? 0 : (c.T1s == 1 ? 1 : 0) + (k - 1);
+}
+
+__device__ inline bool make_search_dev(const Ctl& c, const InstDesc& d, int n, SearchArgs& a) {
+    const int S = c.S;
+    const bool ss = c.method != DMFE_RK54;
+    a = SearchArgs{};
+    a.t1 = d.H.t1;
+    a.tl = c.gt[n];
+    if (n == S) {
+        a.mode = ss ? kSearchI : kSearchSame;
+        a.T1 = c.T1s + 1;
+    } else if (n == 0) {
+        if (c.T1s != 1) return false;
+        a.mode = kSearchB;
+        a.T1 = c.T1s;
+    } else {
+        a.T1 = c.T1s + 1;
+        a.mode = kSearchI;
+        if (ss && n >= 5 && n <= 7) {
+            a.mode = kSearchCopy;
+            a.srcB1 = d.W.saveB1 + (size_t)(n - 5) * d.T.N;
+            a.srcB2 = d.W.saveB2 + (size_t)(n - 5) * d.T.P;
+        } else if (ss && n >= 2 && n <= 4) {
+            a.saveB1 = d.W.saveB1 + (size_t)(n - 2) * d.T.N;
+            a.saveB2 = d.W.saveB2 + (size_t)(n - 2) * d.T.P;
+        }
+    }
+    a.fbuf = (c.fc0 + searches_before(c, n) + 1) & 1;
+    return true;
+}
+
+__device__ inline void make_finish_dev(const Ctl& c, const InstDesc& d, int m, FinishArgs& f) {
+    const int S = c.S;
+    const int t = c.method == DMFE_RK54 ? 0 : 1;
+    const bool last = (m == S - 1);
+    const double dt = c.dt;
+    f = FinishArgs{};
+    f.mode = 1;
+    f.stage = m;
+    f.split = d.split;
+    f.lr = (m == 0) ? c.T1s - 1 : c.T1s;
+    f.g0row = c.T1s - 1;
+    f.wrow = c.T1s;
+    f.t_rhs = c.gt[m];
+    f.t_new = c.gt[m + 1];
+    f.nterms = (m == 0) ? 1 : (last ? S : m + 1);
+    for (int j = 0; j < f.nterms; ++j) f.coef[j] = dt * (last ? cB[t][j] : cA[t][m + 1][j]);
+    f.tdiff = c.gt[m + 1] - c.gt[0];
+    f.rho_new = (c.T1s > 1) ? f.tdiff / (c.gt[0] - c.tprev) : 0.0;
+    f.last = last ? 1 : 0;
+    if (last) {
+        for (int j = 0; j < S; ++j) f.coefE[j] = dt * cB2[t][j];
+        f.gt_err = c.gt_err;
+    }
+    f.fbuf = (c.fc0 + searches_before(c, m + 1)) & 1;
+}
+
+// begin: loop condition, stage times gt[] and the error's time term (runge_kutta.cpp:167,201,207-211,234-238)
+__global__ void batch_begin_kernel(InstDesc* D, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ctl& c = *D[i].ctl;
+    c.active = 0;
+    if (c.status != kRunning) return;
+    const double eps = 2.220446049250313e-16;
+    const double tol = eps * fmax(fabs(c.dt), fabs(c.dt_min));
+    if (!(c.t_prop < c.tmax && c.loop < c.max_loop && c.dt >= c.dt_min - tol)) {
+        c.status = kDone;
+        return;
+    }
+    if (c.T1 + 1 > c.cap) {
+        c.status = kNeedRows;
+        return;
+    }
+    const double* t1 = D[i].H.t1;
+    c.T1s = c.T1;
+    c.gt[0] = t1[c.T1 - 1];
+    c.tprev = c.T1 >= 2 ? t1[c.T1 - 2] : 0.0;
+    c.t_prop = c.gt[0] + c.dt;
+    c.delta_old = c.delta;
+    const int t = c.method == DMFE_RK54 ? 0 : 1;
+    const int S = t == 0 ? 7 : 10;
+    c.S = S;
+    const double dt = c.dt;
+    c.gt[1] = c.gt[0] + dt * cA[t][1][0] * 1.0;
+    for (int m = 1; m < S; ++m) {
+        const bool last = (m == S - 1);
+        double g = c.gt[0];
+        const int nt = last ? S : m + 1;
+        for (int j = 0; j < nt; ++j) g += dt * (last ? cB[t][j] : cA[t][m + 1][j]) * 1.0;
+        c.gt[m + 1] = g;
+    }
+    double gte = c.gt[0];
+    for (int j = 0; j < S; ++j) gte += dt * cB2[t][j] * 1.0;
+    c.gt_err = fabs(c.gt[S] - gte);
+    c.fc0 = c.fcur;
+    c.active = 1;
+}
+
+// end: delta, loop, maintenance cadence and the adaptive dt rule (simulation_runner.cpp:144-278)
+__global__ void batch_end_kernel(InstDesc* D, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ctl& c = *D[i].ctl;
+    if (!c.active) return;  // stays set until the next begin: the observables launch reads it
+    c.delta = D[i].W.scal[1];
+    c.loop += 1;
+    c.T1 = c.T1s + 1;
+    c.fcur = (c.fc0 + searches_before(c, c.S) + 1) & 1;
+    if (c.maint_every > 0 && c.loop % c.maint_every == 0) {
+        c.status = kNeedMaint;  // sparsify, interpolate and the dt rule run on the host
+        return;
+    }
+    const double dmax = c.delta_max;
+    const double rho_last = D[i].H.rho[c.T1 - 1];
+    if (c.delta < dmax && c.loop > 5 && (c.delta < 1.1 * c.delta_old || c.delta_old < dmax / 1000) &&
+        c.rmax[c.method - 1] / c.specRad > c.dt && rho_last == 1.0) {
+        c.dt *= 1.01;
+    } else if (c.delta > 2 * dmax && c.dt > c.dt_min) {
+        c.dt *= 0.9;
+    }
+    if (c.method == DMFE_SSPRK104 && c.delta > dmax && c.rmax[0] / c.specRad > c.dt) {
+        c.method = DMFE_RK54;
+        c.dt *= 0.5;
+    }
+}
+#pragma clang fp contract(on)
+
+// Stage launch n of a batch step: map[b] = (instance, role); role -2 finish of stage
+// n-1, -1 theta-row search, k >= 0 phi2 row k of the interpolation of stage n.
+template <int EPT>
+__global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(const InstDesc* __restrict__ D,
+                                                                    const int2* __restrict__ map, int n) {
+    __shared__ double lds[EPT * kStageThreads];
+    __shared__ double red[8 * (kStageThreads / kWave)];
+    const int2 mp = map[blockIdx.x];
+    if (mp.x < 0) return;
+    const InstDesc& d = D[mp.x];
+    const Ctl& c = *d.ctl;
+    if (!c.active || n > c.S) return;
+    if (mp.y == -2) {
+        if (n >= 1) {
+            FinishArgs f;
+            make_finish_dev(c, d, n - 1, f);
+            finish_block<EPT>(d.T, d.H, d.W, d.M, f, red);
+        }
+        return;
+    }
+    SearchArgs a;
+    if (!make_search_dev(c, d, n, a)) return;
+    if (mp.y == -1) search_theta_row<EPT>(d.T, d.W, a, lds, red);
+    else if (!(a.mode == kSearchSame && !a.saveB2)) search_phi2_row<EPT>(d.T, d.W, a, mp.y, lds);
+}
+
+// RHS launch n: map[b] = (instance, row j, chunk, -); blocks of one (j, chunk) for all
+// instances sharing a grid are adjacent, so their stencil weights are fetched once per XCD.
+template <bool LOGR, int G>
+__global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* __restrict__ D,
+                                                                const int4* __restrict__ map, int n) {
+    extern __shared__ double2 dyn[];
+    __shared__ double red[kSums * (kRowThreads / kWave)];
+    const int4 mp = map[xcd_remap(blockIdx.x, gridDim.x)];
+    if (mp.x < 0) return;
+    const InstDesc& d = D[mp.x];
+    const Ctl& c = *d.ctl;
+    if (!c.active || n >= c.S) return;
+    RhsArgs a{};
+    a.T1 = (n == 0) ? c.T1s : c.T1s + 1;
+    a.tl = c.gt[n];
+    a.split = d.split;
+    a.log_response = LOGR;
+    a.fields = nullptr;
+    a.fbuf = (c.fc0 + searches_before(c, n + 1)) & 1;
+    rhs_rows_body<LOGR, G>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red);
+}
+
+// Observables of the instances that just stepped: (t_prop, E(t_prop), QK(t,0)) per row
+// (simulation_runner.cpp:308-337), appended at row (loop - 1) % cap of obs[inst].
+__global__ __launch_bounds__(kStageThreads) void batch_obs_kernel(const InstDesc* __restrict__ D, double* obs, int cap) {
+    __shared__ double red[2 * (kStageThreads / kWave)];
+    const InstDesc& d = D[blockIdx.x];
+    const Ctl& c = *d.ctl;
+    if (!c.active) return;  // did not step in this batch step
+    const int row = c.T1 - 1;
+    const double t = c.t_prop;
+    const double E = row_scalar_body(d.T, d.H, d.M, row, t, 1, red);
+    if (threadIdx.x == 0) {
+        double* o = obs + ((size_t)blockIdx.x * cap + (size_t)((c.loop - 1) % cap)) * 4;
+        o[0] = t;
+        o[1] = E;
+        o[2] = d.H.V[(size_t)row * d.T.N].x;
+        o[3] = (double)c.loop;
     }
 }
 
@@ -838,6 +1059,56 @@ void launch_gather_rows(const Tables& T, const History& src, const History& dst,
                         hipStream_t s) {
     if (nkeep <= 0) return;
     hipLaunchKernelGGL(gather_rows_kernel, dim3(nkeep), dim3(kRowThreads), 0, s, T, src, dst, keep, nkeep);
+}
+
+
+void batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk) {
+    double A[2][kMaxStages][kMaxStages] = {}, B[2][kMaxStages] = {}, B2[2][kMaxStages] = {};
+    const Tableau* t[2] = {&rk54, &ssprk};
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < kMaxStages; ++i) {
+            for (int j = 0; j < kMaxStages; ++j) A[k][i][j] = t[k]->A[i][j];
+            B[k][i] = t[k]->b[i];
+            B2[k][i] = t[k]->b2[i];
+        }
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(cA), A, sizeof A);
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(cB), B, sizeof B);
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(cB2), B2, sizeof B2);
+}
+
+void launch_batch_begin(InstDesc* D, int n, hipStream_t s) {
+    hipLaunchKernelGGL(batch_begin_kernel, dim3((n + 63) / 64), dim3(64), 0, s, D, n);
+}
+
+void launch_batch_end(InstDesc* D, int n, hipStream_t s) {
+    hipLaunchKernelGGL(batch_end_kernel, dim3((n + 63) / 64), dim3(64), 0, s, D, n);
+}
+
+void launch_batch_stage(InstDesc* D, const int2* map, int nblocks, int n, int ept, hipStream_t s) {
+    if (ept <= 1) hipLaunchKernelGGL(batch_stage_kernel<1>, dim3(nblocks), dim3(kStageThreads), 0, s, D, map, n);
+    else if (ept <= 2) hipLaunchKernelGGL(batch_stage_kernel<2>, dim3(nblocks), dim3(kStageThreads), 0, s, D, map, n);
+    else if (ept <= 4) hipLaunchKernelGGL(batch_stage_kernel<4>, dim3(nblocks), dim3(kStageThreads), 0, s, D, map, n);
+    else hipLaunchKernelGGL(batch_stage_kernel<8>, dim3(nblocks), dim3(kStageThreads), 0, s, D, map, n);
+}
+
+void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_response, int G, size_t lds,
+                      hipStream_t s) {
+#define DMFE_BRHS(LOG, GG) \
+    hipLaunchKernelGGL((batch_rhs_kernel<LOG, GG>), dim3(nblocks), dim3(kRowThreads), lds, s, D, map, n)
+    if (log_response) {
+        if (G <= 8) DMFE_BRHS(true, 8);
+        else if (G <= 16) DMFE_BRHS(true, 16);
+        else DMFE_BRHS(true, 64);
+    } else {
+        if (G <= 8) DMFE_BRHS(false, 8);
+        else if (G <= 16) DMFE_BRHS(false, 16);
+        else DMFE_BRHS(false, 64);
+    }
+#undef DMFE_BRHS
+}
+
+void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t s) {
+    hipLaunchKernelGGL(batch_obs_kernel, dim3(n), dim3(kStageThreads), 0, s, D, obs, obs_cap);
 }
 
 }  // namespace dmfe

@@ -121,7 +121,9 @@ typedef struct dmfe_params {
     double delta_max;     /* -e */
     double rmax[11];      /* stability radii per method (config.hpp:33) */
     int32_t log_response; /* -R, interpolate QR in log space */
-    int32_t reserved;
+    int32_t maint_every;  /* main-loop cadence of sparsify + re-interpolation
+                             (simulation_runner.cpp:161, fixed at 100000 there);
+                             <= 0 means 100000 */
 } dmfe_params;
 
 /* Fills *p with the reference defaults (config.hpp:18-56). */
@@ -214,6 +216,62 @@ int dmfe_copy_probe(int64_t bytes, int reps, double* ms_out);
 
 /* Waits for all work queued on the context's stream. */
 int dmfe_synchronize(dmfe_ctx* ctx);
+
+/* ------------------------------------------------------------ batched quenches */
+/* A batch holds n independent quenches (each its own grid, model, history, dt and
+ * integrator) advanced together: one batch step runs one iteration of the main
+ * loop (simulation_runner.cpp:125-278) for every instance that is still running,
+ * with the scalar step control on the device, so that a step is a fixed sequence
+ * of kernel launches, replayed from a captured hipGraph by dmfe_batch_run.
+ * Instances sharing a grid (same table pointers) share its device tables.  The
+ * every-1e5-loops maintenance (sparsify + interpolate, :161-233) and history
+ * growth run on the host between graph replays.  The reference runs one quench
+ * per process; this has no single counterpart there and replaces a loop of
+ * runSimulation() calls over parameter points. */
+typedef struct dmfe_batch dmfe_batch;
+
+/* Instance status (dmfe_batch_get_control). */
+#define DMFE_BATCH_RUNNING 0
+#define DMFE_BATCH_DONE 1
+
+/* grids[n], params[n]; params[i].log_response must be equal for all i.
+ * rows_cap: initial history rows per instance (grown on demand). */
+int dmfe_batch_create(int n, const dmfe_grid* grids, const dmfe_params* params, int64_t rows_cap,
+                      dmfe_batch** out);
+int dmfe_batch_destroy(dmfe_batch* b);
+int dmfe_batch_size(const dmfe_batch* b, int* n);
+
+/* dmfe_init_quench for every instance. */
+int dmfe_batch_init_quench(dmfe_batch* b);
+
+/* Loop condition of every instance: t < tmax, loop < max_loop, dt >= delta_t_min
+ * (simulation_runner.cpp:125).  Default: unlimited. */
+int dmfe_batch_set_limits(dmfe_batch* b, double tmax, int64_t max_loop);
+
+/* One batch step with direct launches; *running = instances still running. */
+int dmfe_batch_step(dmfe_batch* b, int* running);
+
+/* Up to max_steps batch steps, stopping early when no instance runs.  use_graph:
+ * replay a hipGraph of `chunk` captured steps (host checks between replays);
+ * otherwise launch directly.  *steps_out = batch steps run. */
+int dmfe_batch_run(dmfe_batch* b, int64_t max_steps, int use_graph, int chunk, int64_t* steps_out);
+
+/* Per-instance control and state, as dmfe_get_control / dmfe_set_control /
+ * dmfe_get_T1 / dmfe_get_state. */
+int dmfe_batch_get_control(dmfe_batch* b, int i, double* delta_t, double* delta, double* delta_old, int64_t* loop,
+                           int* method, int* status);
+int dmfe_batch_set_control(dmfe_batch* b, int i, double delta_t, double delta, double delta_old, int64_t loop,
+                           int method);
+int dmfe_batch_get_T1(dmfe_batch* b, int i, int64_t* T1);
+int dmfe_batch_get_state(dmfe_batch* b, int i, double* t1grid, double* QK, double* QR, double* dQK, double* dQR,
+                         double* rvec, double* drvec, double* dtratio);
+
+/* Correlation records (the correlation.txt columns t, E, QK(t,0), plus loop) of every
+ * step, kept on the device in a ring of `cap` rows per instance (row (loop-1) % cap);
+ * cap <= 0 switches recording off.  dmfe_batch_records copies instance i's ring
+ * (cap x 4 doubles). */
+int dmfe_batch_record(dmfe_batch* b, int cap);
+int dmfe_batch_records(dmfe_batch* b, int i, double* out);
 
 #ifdef __cplusplus
 }
