@@ -36,11 +36,17 @@ def algorithmic_bytes_rhs(N, m):
     return (24 * m + 148) * N * N
 
 
+_GRIDS = {}
+
+
 def make_grid(N=512):
     import dynamite_amd as dm
+    if N in _GRIDS:
+        return _GRIDS[N]
     t0 = time.time()
     g = dm.generate_grid(N, 1e5, 5, dm.INTERP_POLY, 9, 0)
     log(f"[bench] grid L={N} m={g.m} generated in {time.time() - t0:.1f}s")
+    _GRIDS[N] = g
     return g
 
 
@@ -70,9 +76,12 @@ def gpu_run(g, prep, steps, warmup):
     el = time.perf_counter() - t0
     prof = {k: dm.profile_read(ctx, k) for k in (0, 1, 2)}
     dm.profile(ctx, False)
+    # the RHS kernel alone, back to back on the final state (per-launch events inside
+    # the steps also count the gap to the preceding stage launch)
+    rhs_b2b_ms = dm.time_rhs(ctx, 200)
     T1 = ctx.T1
     ctx.close()
-    return el, prof, state, control, T1
+    return el, prof, state, control, T1, rhs_b2b_ms
 
 
 def cpu_baseline(g, state, control, budget_s=15.0):
@@ -110,14 +119,95 @@ def cpu_baseline(g, state, control, budget_s=15.0):
                        f"{el:.1f}s wall")
 
 
+BATCH_SEED = 20261203
+
+
+def batch_workload(n=64):
+    """64 quenches: bath temperature Gamma ~ U[0.3, 1.0], lambda ~ U[0, 1], L drawn 50/50
+    from {256, 512} (seeded), p=3, q=4, T0=inf; every instance on its own grid package
+    of the bench grid family (instances of one L share the device tables)."""
+    import numpy as np
+    import dynamite_amd as dm
+    rng = np.random.default_rng(BATCH_SEED)
+    Gam = rng.uniform(0.3, 1.0, n)
+    lam = rng.uniform(0.0, 1.0, n)
+    Ls = rng.permutation(np.array([256] * (n // 2) + [512] * (n - n // 2)))
+    grids = {L: make_grid(int(L)) for L in sorted(set(Ls.tolist()))}
+    gl, pl = [], []
+    for i in range(n):
+        gl.append(grids[int(Ls[i])])
+        pl.append(dm.default_params(p=3, p2=4, lambda_=float(lam[i]), T0=1e50, Gamma=float(Gam[i])))
+    return gl, pl, dict(Gamma=Gam, lam=lam, L=Ls)
+
+
+def batch_run(prep, steps, looped_steps, chunk=16):
+    """instance-steps/s of the batch (hipGraph replays) and of the plain path looped over
+    the same instances (one context per instance, its steps one after another), both
+    from the same live states after `prep` batch steps."""
+    import numpy as np
+    import dynamite_amd as dm
+    gl, pl, meta = batch_workload()
+    n = len(gl)
+    b = dm.Batch(gl, pl, rows_cap=prep + steps + 64)
+    b.init_quench()
+    t0 = time.perf_counter()
+    b.run(prep, use_graph=True, chunk=chunk)
+    log(f"[bench] batch of {n}: {prep} prep steps in {time.perf_counter() - t0:.2f}s, "
+        f"T1 {min(b.T1(i) for i in range(n))}..{max(b.T1(i) for i in range(n))}")
+    b.run(2 * chunk, use_graph=True, chunk=chunk)  # warm-up, graph instantiation
+    loops0 = np.array([b.get_control(i)["loop"] for i in range(n)])
+    states = [(b.get_state(i), b.get_control(i)) for i in range(n)]
+    t0 = time.perf_counter()
+    nb = b.run(steps, use_graph=True, chunk=chunk)
+    el = time.perf_counter() - t0
+    loops1 = np.array([b.get_control(i)["loop"] for i in range(n)])
+    inst_steps = int((loops1 - loops0).sum())
+    b.close()
+    batch_rate = inst_steps / el
+    log(f"[bench] batch: {nb} batch steps, {inst_steps} instance-steps in {el:.3f}s -> {batch_rate:.0f}/s "
+        f"({1e3 * el / max(nb, 1):.2f} ms per batch step)")
+    # looped plain path on the same states
+    ctxs = []
+    for i in range(n):
+        st, c = states[i]
+        ctx = dm.Context(gl[i], pl[i], rows_hint=len(st["t1grid"]) + looped_steps + 64)
+        ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
+        ctx.set_control(c["delta_t"], c["delta"], c["delta_old"], c["loop"], c["method"])
+        ctx.interpolate(dm.POS_BSEARCH)
+        ctx.loop_iteration()  # warm-up
+        ctx.synchronize()
+        ctxs.append(ctx)
+    t0 = time.perf_counter()
+    for ctx in ctxs:
+        for _ in range(looped_steps):
+            ctx.loop_iteration()
+        ctx.synchronize()
+    el_l = time.perf_counter() - t0
+    for ctx in ctxs:
+        ctx.close()
+    looped_rate = n * looped_steps / el_l
+    log(f"[bench] looped plain path: {n} x {looped_steps} steps in {el_l:.3f}s -> {looped_rate:.0f}/s")
+    return dict(instances=n, value=batch_rate, unit="instance-steps/s", looped_value=looped_rate,
+                ratio=batch_rate / looped_rate, batch_steps=nb, instance_steps=inst_steps,
+                ms_per_batch_step=1e3 * el / max(nb, 1), prep_steps=prep, graph_chunk=chunk,
+                workload=f"{n} quenches, Gamma~U[0.3,1], lambda~U[0,1], L 50/50 in {{256,512}} "
+                         f"(seed {BATCH_SEED}), p=3 q=4 T0=inf, poly-9 grids (spline 5, Tmax 1e5); "
+                         f"timed from the live states after {prep} batch steps")
+
+
 def read_traffic():
-    """HBM bytes per rhs_rows_kernel launch from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", "rhs_rows_L512_pmc.json")
+    """HBM bytes per rhs_rows_kernel launch on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/rhs_rows_kernel_L512_hbm.txt: the line
+    `traffic_bytes_per_launch = ...`, counters scaled by the plain-copy calibration)."""
+    path = os.path.join(ROOT, "profiles", "rhs_rows_kernel_L512_hbm.txt")
     try:
         with open(path) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except Exception:
-        return None
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -128,6 +218,11 @@ def main():
     ap.add_argument("--prep", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-prep", type=int, default=1000)
+    ap.add_argument("--batch-steps", type=int, default=64)
+    ap.add_argument("--looped-steps", type=int, default=10)
+    ap.add_argument("--batch-only", action="store_true", help="only the batch-64 measurement")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain HBM copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args()
@@ -142,16 +237,22 @@ def main():
         print(f"bench.py: this build runs on one GPU only (got --gpus {a.gpus}); "
               "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
         sys.exit(2)
+    if a.batch_only:
+        out = batch_run(a.batch_prep, a.batch_steps, a.looped_steps)
+        print(json.dumps({"batch": out}), flush=True)
+        return
     N = 512
     g = make_grid(N)
     log(f"[bench] GPU: {a.prep} prep + {a.warmup} warmup + {a.steps} timed steps")
-    el, prof, state, control, T1 = gpu_run(g, a.prep, a.steps, a.warmup)
+    el, prof, state, control, T1, rhs_b2b_ms = gpu_run(g, a.prep, a.steps, a.warmup)
     ms_step = 1e3 * el / a.steps
     rhs_ms, rhs_n = prof[1]
-    avg_rhs_s = rhs_ms / 1e3 / max(rhs_n, 1)
+    avg_rhs_in_steps_s = rhs_ms / 1e3 / max(rhs_n, 1)
+    avg_rhs_s = rhs_b2b_ms / 1e3
     B = algorithmic_bytes_rhs(N, g.m)
     achieved = B / avg_rhs_s / 1e9
-    log(f"[bench] {ms_step:.3f} ms/step; rhs_rows {rhs_n} launches avg {avg_rhs_s * 1e6:.1f} us "
+    log(f"[bench] {ms_step:.3f} ms/step; rhs_rows {rhs_n} launches avg {avg_rhs_in_steps_s * 1e6:.1f} us in steps, "
+        f"{avg_rhs_s * 1e6:.1f} us back to back "
         f"-> {achieved:.0f} GB/s algorithmic; stage (finish+search) avg "
         f"{prof[2][0] / max(prof[2][1], 1) * 1e3:.1f} us x {prof[2][1]}; search-only avg "
         f"{prof[0][0] / max(prof[0][1], 1) * 1e3:.1f} us x {prof[0][1]}")
@@ -176,8 +277,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "rhs_rows_kernel", "bytes_per_launch": B,
-                     "avg_launch_us": avg_rhs_s * 1e6, "launches": rhs_n},
+                     "avg_launch_us": avg_rhs_s * 1e6, "avg_launch_us_in_steps": avg_rhs_in_steps_s * 1e6,
+                     "launches_in_steps": rhs_n},
     }
+    if not a.no_batch:
+        out["batch"] = batch_run(a.batch_prep, a.batch_steps, a.looped_steps)
     if not a.no_cpu:
         log(f"[bench] CPU baseline (budget {a.cpu_budget:.0f}s)")
         out["cpu_baseline"] = cpu_baseline(g, state, control, a.cpu_budget)

@@ -295,6 +295,13 @@ def profile_read(ctx, kind):
     return ms.value, n.value
 
 
+def time_rhs(ctx, reps=100):
+    """Average ms of `reps` back-to-back RHS kernel launches on the context's current state."""
+    ms = C.c_double()
+    _check(lib().dmfe_time_rhs(ctx._h, C.c_int(reps), C.byref(ms)), "dmfe_time_rhs")
+    return ms.value
+
+
 def copy_probe(nbytes, reps=10):
     """Average ms of a plain HBM copy of nbytes (counter calibration)."""
     ms = C.c_double()

@@ -833,6 +833,30 @@ extern "C" int dmfe_profile_read(dmfe_ctx* c, int kind, double* total_ms, int64_
     return DMFE_OK;
 }
 
+extern "C" int dmfe_time_rhs(dmfe_ctx* c, int reps, double* ms_out) {
+    if (!c || reps < 1 || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_time_rhs: bad argument");
+    if (!c->interp_valid) return set_error(DMFE_ERR_STATE, "dmfe_time_rhs: no interpolation state");
+    const bool was = c->profiling;
+    c->profiling = false;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = run_rhs_rows(c, nullptr);  // warm
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps && rc == DMFE_OK; ++r) rc = run_rhs_rows(c, nullptr);
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->profiling = was;
+    if (rc) return rc;
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("dmfe_time_rhs: ") + hipGetErrorString(e));
+    if (ms_out) *ms_out = ms / reps;
+    return DMFE_OK;
+}
+
 extern "C" int dmfe_copy_probe(int64_t bytes, int reps, double* ms_out) {
     if (bytes < 16 || reps < 1) return set_error(DMFE_ERR_ARG, "dmfe_copy_probe: bad argument");
     const long long n = bytes / 16;

@@ -152,6 +152,9 @@ struct Ctl {
     double gt[kMaxStages + 1];
     double tprev;       // t1grid[T1s - 2]
     double gt_err;      // |gt_S - gte|
+    // launch arguments of this step, written by the begin kernel
+    FinishArgs fin[kMaxStages];      // finish of stage m
+    SearchArgs srch[kMaxStages + 1]; // searches of stage n (mode < 0: none)
 };
 
 struct InstDesc {

@@ -65,22 +65,70 @@ __device__ inline double wave_max(double v) {
 
 // Sum over aligned groups of G lanes (G in 4, 8, 16, 32, 64) with DPP lane swizzles
 // (quad_perm, row_half_mirror, row_mirror) and a cross-row shuffle for G >= 32.
+// Every control used here reads an active lane of the same 16-lane row, so the
+// destination needs no initial value.
 template <int CTRL>
 __device__ inline double dpp_f64(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int kDppQuadSwap1 = 0xB1;     // quad_perm [1,0,3,2]: lane i <-> i^1
+constexpr int kDppQuadSwap2 = 0x4E;     // quad_perm [2,3,0,1]: lane i <-> i^2
+constexpr int kDppHalfMirror = 0x141;   // row_half_mirror: lane i <-> 7-i within 8
+constexpr int kDppRowMirror = 0x140;    // row_mirror: lane i <-> 15-i within 16
+// row_ror:n makes lane r of a 16-lane row read lane (r - n) mod 16
+constexpr int kDppRowRor8 = 0x128;      // row_ror:8: lane r <- (r+8) mod 16 within the row
+
+// Reduce-scatter of eight per-lane values over an aligned group of 8 lanes: on return
+// lane (i mod 8) holds the group total of value (i mod 8).  Partners: 7-i, i^2, i^1.
+__device__ inline double scatter8(const double (&v)[8], int i8) {
+    const bool h = i8 & 4, b = i8 & 2, c = i8 & 1;
+    double k4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double keep = h ? v[4 + q] : v[q], send = h ? v[q] : v[4 + q];
+        k4[q] = keep + dpp_f64<kDppHalfMirror>(send);
+    }
+    double k2[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double keep = b ? k4[2 + q] : k4[q], send = b ? k4[q] : k4[2 + q];
+        k2[q] = keep + dpp_f64<kDppQuadSwap2>(send);
+    }
+    const double keep = c ? k2[1] : k2[0], send = c ? k2[0] : k2[1];
+    return keep + dpp_f64<kDppQuadSwap1>(send);
 }
 template <int G>
 __device__ inline double group_sum(double v) {
-    v += dpp_f64<0xB1>(v);  // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
-    if (G >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror: lane i <-> 7-i within 8
-    if (G >= 16) v += dpp_f64<0x140>(v);  // row_mirror: lane i <-> 15-i within 16
+    v += dpp_f64<kDppQuadSwap1>(v);
+    v += dpp_f64<kDppQuadSwap2>(v);
+    if (G >= 8) v += dpp_f64<kDppHalfMirror>(v);
+    if (G >= 16) v += dpp_f64<kDppRowMirror>(v);
     if (G >= 32) v += __shfl_xor(v, 16, kWave);
     if (G >= 64) v += __shfl_xor(v, 32, kWave);
     return v;
+}
+
+// Sums eight values over the block: scatter8 inside groups of 8 lanes, then the
+// partners r+8 (row_ror:8), lane^16 and lane^32; thread q < 8 returns total q.
+// Fixed order.
+template <int NT>
+__device__ inline double block_sum8(const double (&v)[8], double* lds /* 8*NT/64 */) {
+    constexpr int NW = NT / kWave;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    double t = scatter8(v, lane & 7);
+    t += dpp_f64<kDppRowRor8>(t);
+    t += __shfl_xor(t, 16, kWave);
+    t += __shfl_xor(t, 32, kWave);
+    if (lane < 8) lds[lane * NW + wid] = t;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x < 8)
+        for (int w = 0; w < NW; ++w) s += lds[threadIdx.x * NW + w];
+    __syncthreads();
+    return s;
 }
 
 // Sums K values over the block; every thread gets the totals in out[]. Fixed order.
@@ -188,7 +236,7 @@ __device__ inline size_t bracket_of(double pos, size_t L) {
 // from the previous positions (init), which are usually already exact, so most
 // rows finish in one or two sweeps.
 template <int EPT>
-__device__ void search_row(const ListView& list, double tl, const double* __restrict__ elem, const double* init,
+__device__ __forceinline__ void search_row(const ListView& list, double tl, const double* __restrict__ elem, const double* init,
                            double* out, double* out2, int len, int mode, double* lds) {
     const int tid = threadIdx.x;
     double El[EPT], ini[EPT], res[EPT];
@@ -247,7 +295,7 @@ __device__ void search_row(const ListView& list, double tl, const double* __rest
 
 // B2 positions of phi2 row k (interpolation_core.cpp:27-29).
 template <int EPT>
-__device__ void search_phi2_row(const Tables& T, const Work& W, const SearchArgs& a, int k, double* lds) {
+__device__ __forceinline__ void search_phi2_row(const Tables& T, const Work& W, const SearchArgs& a, int k, double* lds) {
     const int N = T.N;
     double* out = W.posB2 + (size_t)k * N;
     double* out2 = a.saveB2 ? a.saveB2 + (size_t)k * N : nullptr;
@@ -271,7 +319,7 @@ __device__ void search_phi2_row(const Tables& T, const Work& W, const SearchArgs
 // B1 positions of the theta row and the B1 prologue Floor / diff
 // (interpolation_core.cpp:22-24, 57-84).
 template <int EPT>
-__device__ void search_theta_row(const Tables& T, const Work& W, const SearchArgs& a, double* lds, double* red) {
+__device__ __forceinline__ void search_theta_row(const Tables& T, const Work& W, const SearchArgs& a, double* lds, double* red) {
     const int N = T.N;
     double* pos = W.posB1;
     if (a.mode == kSearchCopy) {
@@ -366,6 +414,71 @@ __device__ inline double2 stencil_last(const double* __restrict__ w, long long P
 }
 #pragma clang fp contract(on)
 
+// B2 (index_mat.cpp:18-91), generic form: groups of G lanes per entry, lane d holds
+// taps d, d+G, ...; gathers of one wave instruction touch 64/G entries x a few
+// contiguous cache lines; the m-tap sums are closed by group_sum and the time
+// Hermite is applied by the group's first lane.
+template <bool LOGR, int G>
+__device__ __forceinline__ void b2_groups(const Tables& T, const History& H, const Work& W, const RhsArgs& a, int j,
+                                          int c0, int C, const int* b2ix, const int* b2s, const double* b2u,
+                                          double2* b2) {
+    const int N = T.N, m = T.m, T1 = a.T1;
+    constexpr int GPW = kWave / G;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int g = lane / G, d0 = lane % G;
+    constexpr int kWaves = kRowThreads / kWave;
+    for (int e0 = wid * GPW; e0 < C; e0 += GPW * kWaves) {
+        const int e = e0 + g;
+        const bool ev = e < C;
+        const int ee = ev ? e : 0;
+        const size_t ix = (size_t)b2ix[ee];
+        const int sst = b2s[ee];
+        const bool b2int = ix < (size_t)(T1 - 1);
+        const double2* R0 = H.V + (ix - 1) * N + sst;
+        const double2* R1 = H.V + ix * N + sst;
+        const double2* S1 = H.D + ix * N + sst;
+        const double2* S2 = H.D + (b2int ? ix + 1 : ix) * N + sst;
+        const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
+        double acc8[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc8[q] = 0.0;
+        if (ev) {
+            for (int dd = d0; dd < m; dd += G) {
+                const double wd = wk[dd];
+                const double2 v0 = R0[dd], v1 = R1[dd], g1 = S1[dd], g2 = S2[dd];
+                acc8[0] += wd * v0.x; acc8[1] += wd * v0.y;
+                acc8[2] += wd * v1.x; acc8[3] += wd * v1.y;
+                acc8[4] += wd * g1.x; acc8[5] += wd * g1.y;
+                acc8[6] += wd * g2.x; acc8[7] += wd * g2.y;
+            }
+        }
+        const double y0k = group_sum<G>(acc8[0]), y0r = group_sum<G>(acc8[1]);
+        const double y1k = group_sum<G>(acc8[2]), y1r = group_sum<G>(acc8[3]);
+        const double e1k = group_sum<G>(acc8[4]), e1r = group_sum<G>(acc8[5]);
+        const double e2k = group_sum<G>(acc8[6]), e2r = group_sum<G>(acc8[7]);
+        if (d0 == 0 && ev) {
+            const double u = b2u[e], u2 = u * u, u3 = u2 * u;
+            double qk, qr;
+            if (b2int) {
+                const double rr = H.rho[ix + 1];
+                const double h0 = 1 - 3 * u2 + 2 * u3, h1 = u - 2 * u2 + u3, h2 = 3 * u2 - 2 * u3, h3 = -u2 + u3;
+                qk = h0 * y0k + h1 * e1k + h2 * y1k + h3 * e2k / rr;
+                if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
+                    qr = h0 * y0r + h1 * e1r + h2 * y1r + h3 * e2r / rr;
+                else
+                    qr = exp(h0 * log(y0r) + h1 * (e1r / y0r) + h2 * log(y1r) + (h3 / rr) * (e2r / y1r));
+            } else {
+                qk = (1 - u2) * y0k + u2 * y1k + (u - u2) * e1k;
+                if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
+                    qr = (1 - u2) * y0r + u2 * y1r + (u - u2) * e1r;
+                else
+                    qr = exp((1 - u2) * log(y0r) + u2 * log(y1r) + (u - u2) * (e1r / y0r));
+            }
+            b2[e] = make_double2(qk, qr);
+        }
+    }
+}
+
 // Row j, columns [c0, c1): the eight quadratures of QKstep/QRstep.
 //  phase 1 (B2, index_mat.cpp:18-91): one group of G lanes per target entry, lane d
 //          holding stencil tap d, so the four history-row gathers of a wave touch a
@@ -376,7 +489,7 @@ __device__ inline double2 stencil_last(const double* __restrict__ w, long long P
 //          Hermite (index_vec.cpp:63-136), the self-energies and the weighted
 //          products, then one block reduction.
 template <bool LOGR, int G>
-__device__ void rhs_rows_body(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
+__device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
                               int j, int chunk, double2* dyn /* [N] last row, [cw] (QKB2, QRB2), [cw] u, [cw] ix, s */,
                               double* red) {
     const int N = T.N, m = T.m;
@@ -413,60 +526,7 @@ __device__ void rhs_rows_body(const Tables& T, const History& H, const Work& W, 
         __syncthreads();
         // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
         // instruction touch 64/G entries x a few contiguous cache lines
-        constexpr int GPW = kWave / G;
-        const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-        const int g = lane / G, d0 = lane % G;
-        constexpr int kWaves = kRowThreads / kWave;
-        for (int e0 = wid * GPW; e0 < C; e0 += GPW * kWaves) {
-            const int e = e0 + g;
-            const bool ev = e < C;
-            const int ee = ev ? e : 0;
-            const size_t ix = (size_t)b2ix[ee];
-            const int sst = b2s[ee];
-            const bool b2int = ix < (size_t)(T1 - 1);
-            const double2* R0 = H.V + (ix - 1) * N + sst;
-            const double2* R1 = H.V + ix * N + sst;
-            const double2* S1 = H.D + ix * N + sst;
-            const double2* S2 = H.D + (b2int ? ix + 1 : ix) * N + sst;
-            const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
-            double acc8[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) acc8[q] = 0.0;
-            if (ev) {
-                for (int dd = d0; dd < m; dd += G) {
-                    const double wd = wk[dd];
-                    const double2 v0 = R0[dd], v1 = R1[dd], g1 = S1[dd], g2 = S2[dd];
-                    acc8[0] += wd * v0.x; acc8[1] += wd * v0.y;
-                    acc8[2] += wd * v1.x; acc8[3] += wd * v1.y;
-                    acc8[4] += wd * g1.x; acc8[5] += wd * g1.y;
-                    acc8[6] += wd * g2.x; acc8[7] += wd * g2.y;
-                }
-            }
-            const double y0k = group_sum<G>(acc8[0]), y0r = group_sum<G>(acc8[1]);
-            const double y1k = group_sum<G>(acc8[2]), y1r = group_sum<G>(acc8[3]);
-            const double e1k = group_sum<G>(acc8[4]), e1r = group_sum<G>(acc8[5]);
-            const double e2k = group_sum<G>(acc8[6]), e2r = group_sum<G>(acc8[7]);
-            if (d0 == 0 && ev) {
-                const double u = b2u[e], u2 = u * u, u3 = u2 * u;
-                double qk, qr;
-                if (b2int) {
-                    const double rr = H.rho[ix + 1];
-                    const double h0 = 1 - 3 * u2 + 2 * u3, h1 = u - 2 * u2 + u3, h2 = 3 * u2 - 2 * u3, h3 = -u2 + u3;
-                    qk = h0 * y0k + h1 * e1k + h2 * y1k + h3 * e2k / rr;
-                    if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
-                        qr = h0 * y0r + h1 * e1r + h2 * y1r + h3 * e2r / rr;
-                    else
-                        qr = exp(h0 * log(y0r) + h1 * (e1r / y0r) + h2 * log(y1r) + (h3 / rr) * (e2r / y1r));
-                } else {
-                    qk = (1 - u2) * y0k + u2 * y1k + (u - u2) * e1k;
-                    if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
-                        qr = (1 - u2) * y0r + u2 * y1r + (u - u2) * e1r;
-                    else
-                        qr = exp((1 - u2) * log(y0r) + u2 * log(y1r) + (u - u2) * (e1r / y0r));
-                }
-                b2[e] = make_double2(qk, qr);
-            }
-        }
+        b2_groups<LOGR, G>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2);
     }
     __syncthreads();
 
@@ -541,8 +601,8 @@ __device__ void rhs_rows_body(const Tables& T, const History& H, const Work& W, 
         acc[6] += wi * sRA2 * qrB2;  // ConvR(SigmaRA2, QRB2)
         acc[7] += wi * qrA2 * sRB2;  // ConvR(QRA2, SigmaRB2)
     }
-    block_sum<kSums, kRowThreads>(acc, red);
-    if (threadIdx.x < kSums) W.partial[((size_t)j * a.split + chunk) * kSums + threadIdx.x] = acc[threadIdx.x];
+    const double tot = block_sum8<kRowThreads>(acc, red);
+    if (threadIdx.x < kSums) W.partial[((size_t)j * a.split + chunk) * kSums + threadIdx.x] = tot;
 }
 
 template <bool LOGR, int G>
@@ -555,7 +615,7 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History
 
 // ---- finish: RHS epilogue + RK stage -------------------------------------------------
 template <int EPT>
-__device__ void finish_block(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
+__device__ __forceinline__ void finish_block(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
                              double* red /* 8 * kStageThreads / 64 */) {
     const int N = T.N;
     const int n = a.stage;
@@ -642,7 +702,10 @@ __device__ void finish_block(const Tables& T, const History& H, const Work& W, c
         if (j >= N) continue;
         const double2 g = g0[j];
         double gK = g.x, gR = g.y;
-        for (int q = 0; q < a.nterms; ++q) {
+        // static trip count: a.coef stays in registers when `a` is a local copy
+#pragma unroll
+        for (int q = 0; q < kMaxStages; ++q) {
+            if (q >= a.nterms) break;
             const double hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
             const double hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
             gK = gK + hq * a.coef[q];
@@ -660,7 +723,9 @@ __device__ void finish_block(const Tables& T, const History& H, const Work& W, c
         if (j == 0) { gK0 = gK; sKg0 = sK; }
         if (a.last) {
             double eK = g.x, eR = g.y;
-            for (int q = 0; q <= n; ++q) {
+#pragma unroll
+            for (int q = 0; q < kMaxStages; ++q) {
+                if (q > n) break;
                 const double hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
                 const double hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
                 eK = eK + hq * a.coefE[q];
@@ -810,7 +875,10 @@ __device__ inline bool make_search_dev(const Ctl& c, const InstDesc& d, int n, S
         a.mode = ss ? kSearchI : kSearchSame;
         a.T1 = c.T1s + 1;
     } else if (n == 0) {
-        if (c.T1s != 1) return false;
+        if (c.T1s != 1) {
+            a.mode = -1;
+            return false;
+        }
         a.mode = kSearchB;
         a.T1 = c.T1s;
     } else {
@@ -844,12 +912,14 @@ __device__ inline void make_finish_dev(const Ctl& c, const InstDesc& d, int m, F
     f.t_rhs = c.gt[m];
     f.t_new = c.gt[m + 1];
     f.nterms = (m == 0) ? 1 : (last ? S : m + 1);
-    for (int j = 0; j < f.nterms; ++j) f.coef[j] = dt * (last ? cB[t][j] : cA[t][m + 1][j]);
+#pragma unroll
+    for (int j = 0; j < kMaxStages; ++j) f.coef[j] = (j < f.nterms) ? dt * (last ? cB[t][j] : cA[t][m + 1][j]) : 0.0;
     f.tdiff = c.gt[m + 1] - c.gt[0];
     f.rho_new = (c.T1s > 1) ? f.tdiff / (c.gt[0] - c.tprev) : 0.0;
     f.last = last ? 1 : 0;
     if (last) {
-        for (int j = 0; j < S; ++j) f.coefE[j] = dt * cB2[t][j];
+#pragma unroll
+        for (int j = 0; j < kMaxStages; ++j) f.coefE[j] = (j < S) ? dt * cB2[t][j] : 0.0;
         f.gt_err = c.gt_err;
     }
     f.fbuf = (c.fc0 + searches_before(c, m + 1)) & 1;
@@ -894,6 +964,8 @@ __global__ void batch_begin_kernel(InstDesc* D, int n) {
     for (int j = 0; j < S; ++j) gte += dt * cB2[t][j] * 1.0;
     c.gt_err = fabs(c.gt[S] - gte);
     c.fc0 = c.fcur;
+    for (int k = 0; k <= S; ++k) make_search_dev(c, D[i], k, c.srch[k]);
+    for (int k = 0; k < S; ++k) make_finish_dev(c, D[i], k, c.fin[k]);
     c.active = 1;
 }
 
@@ -939,15 +1011,11 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(const InstDe
     const Ctl& c = *d.ctl;
     if (!c.active || n > c.S) return;
     if (mp.y == -2) {
-        if (n >= 1) {
-            FinishArgs f;
-            make_finish_dev(c, d, n - 1, f);
-            finish_block<EPT>(d.T, d.H, d.W, d.M, f, red);
-        }
+        if (n >= 1) finish_block<EPT>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
         return;
     }
-    SearchArgs a;
-    if (!make_search_dev(c, d, n, a)) return;
+    const SearchArgs& a = c.srch[n];
+    if (a.mode < 0) return;
     if (mp.y == -1) search_theta_row<EPT>(d.T, d.W, a, lds, red);
     else if (!(a.mode == kSearchSame && !a.saveB2)) search_phi2_row<EPT>(d.T, d.W, a, mp.y, lds);
 }

@@ -1,0 +1,55 @@
+"""Short text summaries of rocprofv3 output databases (rocpd SQLite).
+
+    python -m dynamite_amd.rocprof_summary trace <dir> [last_k]
+        per-kernel dispatch count and duration statistics (kernel trace), optionally
+        over the last `last_k` dispatches of each kernel;
+    python -m dynamite_amd.rocprof_summary pmc <dir> [last_k]
+        mean of every collected counter per kernel over its last `last_k` dispatches.
+<dir> is the rocprofv3 -d directory (searched recursively for *.db).
+"""
+import glob
+import sqlite3
+import statistics as st
+import sys
+
+
+def _db(path):
+    found = glob.glob(path + "/**/*.db", recursive=True)
+    if not found:
+        raise SystemExit(f"no rocpd database under {path}")
+    return sqlite3.connect(found[0])
+
+
+def _short(name):
+    s = name.replace("dmfe::(anonymous namespace)::", "").replace("dmfe::", "")
+    s = s.split("(")[0]
+    return s[5:] if s.startswith("void ") else s
+
+
+def trace(path, k):
+    con = _db(path)
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    nm = "name" if "name" in cols else "kernel_name"
+    agg = {}
+    for name, a, b in con.execute(f"select {nm}, start, end from kernels order by start"):
+        agg.setdefault(name, []).append((b - a) / 1e3)
+    print(f"{'kernel':58s} {'calls':>6s} {'total_us':>11s} {'avg_us':>8s} {'min_us':>8s} {'max_us':>8s}")
+    for name, ds in sorted(agg.items(), key=lambda kv: -sum(kv[1][-k:])):
+        d = ds[-k:]
+        print(f"{_short(name)[:58]:58s} {len(d):6d} {sum(d):11.1f} {st.mean(d):8.2f} {min(d):8.2f} {max(d):8.2f}")
+
+
+def pmc(path, k):
+    con = _db(path)
+    agg = {}
+    for name, c, v in con.execute("select kernel_name, counter_name, value from counters_collection order by dispatch_id"):
+        agg.setdefault((name, c), []).append(v)
+    for (name, c), vs in sorted(agg.items()):
+        last = vs[-k:]
+        print(f"{_short(name)[:58]:58s} {c:14s} dispatches={len(vs):6d} last={len(last):5d} mean={st.mean(last):14.1f}")
+
+
+if __name__ == "__main__":
+    mode, path = sys.argv[1], sys.argv[2]
+    k = int(sys.argv[3]) if len(sys.argv) > 3 else 10**9
+    {"trace": trace, "pmc": pmc}[mode](path, k)

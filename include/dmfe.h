@@ -209,6 +209,12 @@ int dmfe_sparsify(dmfe_ctx* ctx, double threshold);
 int dmfe_profile(dmfe_ctx* ctx, int enable);
 int dmfe_profile_read(dmfe_ctx* ctx, int kind, double* total_ms, int64_t* launches);
 
+/* Measurement utility (no counterpart in the reference): `reps` back-to-back
+ * launches of the RHS kernel (rhs_rows_kernel) on the current interpolation state,
+ * timed with two HIP events on the context's stream; the state is not changed.
+ * *ms_out = average per launch in ms. */
+int dmfe_time_rhs(dmfe_ctx* ctx, int reps, double* ms_out);
+
 /* Measurement utility (no counterpart in the reference): `reps` launches of a
  * plain device-to-device copy of `bytes` bytes (hbm_copy_kernel), used to
  * calibrate HBM byte counters; *ms_out = average launch time in ms. */
