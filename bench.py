@@ -119,6 +119,67 @@ def cpu_baseline(g, state, control, budget_s=15.0):
                        f"{el:.1f}s wall")
 
 
+def kernel_microbench(g, reps=200):
+    """RHS and interpolation kernels on the seeded synthetic 512 x 512 history
+    (QK, QR = exp(-|theta_k - theta_j| u_k) (1 + 1e-3 xi), u ~ U[0.5, 2], rows at
+    t_k = 1e3 theta_k; dynamite_amd.workloads.synthetic_history)."""
+    import dynamite_amd as dm
+    from dynamite_amd.workloads import synthetic_history
+    st = synthetic_history(g.theta)
+    prm = dm.default_params(p=PARAMS["p"], p2=PARAMS["p2"], lambda_=PARAMS["lam"], T0=PARAMS["T0"],
+                            Gamma=PARAMS["Gamma"])
+    ctx = dm.Context(g, prm, rows_hint=len(st["t1grid"]) + 8)
+    ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
+    ctx.interpolate(dm.POS_BSEARCH)
+    ctx.synchronize()
+    dm.profile(ctx, True)
+    for _ in range(20):
+        ctx.interpolate(dm.POS_ISEARCH)
+    ctx.synchronize()
+    s_ms, s_n = dm.profile_read(ctx, 0)
+    dm.profile(ctx, False)
+    rhs_ms = dm.time_rhs(ctx, reps)
+    ctx.close()
+    B = algorithmic_bytes_rhs(g.N, g.m)
+    return dict(workload=f"synthetic {len(st['t1grid'])} x {g.N} history (seed {BATCH_SEED}), rows at "
+                         f"t = 1e3 theta; L={g.N} poly-9 grid",
+                rhs_us=rhs_ms * 1e3, rhs_achieved_GBs=B / (rhs_ms * 1e-3) / 1e9,
+                rhs_frac=B / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                interpolation_us=s_ms / max(s_n, 1) * 1e3)
+
+
+def dense_microbench(N=512, order=9, B=4, reps=10):
+    """Dense B-spline path: L=512 grid with interp-method bspline degree 9 (m = N), the
+    A1 table applied to B=4 right-hand sides (QK and QR of the last two rows of the
+    seeded synthetic history) as one (N^2 x N) . (N x B) FP64 MFMA contraction; plus one
+    full RHS evaluation on the dense grid (A1/A2 on MFMA, B2 as the 64-lane stencil)."""
+    import numpy as np
+    import dynamite_amd as dm
+    from dynamite_amd.workloads import synthetic_history
+    t0 = time.time()
+    g = dm.generate_grid(N, 1e5, 5, dm.INTERP_BSPLINE, order, 0)
+    log(f"[bench] dense grid L={N} bspline {order} generated in {time.time() - t0:.1f}s")
+    st = synthetic_history(g.theta)
+    prm = dm.default_params(p=PARAMS["p"], p2=PARAMS["p2"], lambda_=PARAMS["lam"], T0=PARAMS["T0"],
+                            Gamma=PARAMS["Gamma"])
+    ctx = dm.Context(g, prm, rows_hint=len(st["t1grid"]) + 8)
+    ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
+    ctx.interpolate(dm.POS_BSEARCH)
+    QK, QR = st["QK"].reshape(-1, N), st["QR"].reshape(-1, N)
+    X = np.stack([QK[-1], QR[-1], QK[-2], QR[-2]], axis=1)[:, :B]
+    _, ms = dm.dense_apply(ctx, X, 0, reps)
+    rhs_ms = dm.time_rhs(ctx, 5)
+    ctx.close()
+    P = N * N
+    bytes_ = 8.0 * P * N + 8.0 * N * B + 8.0 * P * B
+    flops = 2.0 * P * N * B
+    return dict(workload=f"L={N} bspline degree {order} (m = N), A1 table x {B} right-hand sides "
+                         f"(QK, QR of the last two synthetic rows)",
+                contraction_us=ms * 1e3, bound="hbm", achieved_GBs=bytes_ / (ms * 1e-3) / 1e9,
+                frac=bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, achieved_TFLOPs=flops / (ms * 1e-3) / 1e12,
+                fp64_mfma_peak_TFLOPs=78.6, dense_rhs_us=rhs_ms * 1e3)
+
+
 BATCH_SEED = 20261203
 
 
@@ -219,6 +280,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-dense", action="store_true")
     ap.add_argument("--batch-prep", type=int, default=1000)
     ap.add_argument("--batch-steps", type=int, default=64)
     ap.add_argument("--looped-steps", type=int, default=10)
@@ -280,6 +342,11 @@ def main():
                      "avg_launch_us": avg_rhs_s * 1e6, "avg_launch_us_in_steps": avg_rhs_in_steps_s * 1e6,
                      "launches_in_steps": rhs_n},
     }
+    out["kernel_microbench"] = kernel_microbench(g)
+    log(f"[bench] microbench: {out['kernel_microbench']}")
+    if not a.no_dense:
+        out["dense_bspline"] = dense_microbench()
+        log(f"[bench] dense: {out['dense_bspline']}")
     if not a.no_batch:
         out["batch"] = batch_run(a.batch_prep, a.batch_steps, a.looped_steps)
     if not a.no_cpu:

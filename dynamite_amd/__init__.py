@@ -302,6 +302,20 @@ def time_rhs(ctx, reps=100):
     return ms.value
 
 
+def dense_apply(ctx, X, map_=0, reps=1):
+    """Y = W_map . X for a dense (B-spline) grid context on MFMA; X is (N, B) with B <= 16.
+    Returns (Y of shape (N*N, B), average device ms per contraction)."""
+    X = np.ascontiguousarray(X, np.float64)
+    if X.ndim == 1:
+        X = X[:, None]
+    B = X.shape[1]
+    Y = np.zeros((ctx.N * ctx.N, B))
+    ms = C.c_double()
+    _check(lib().dmfe_dense_apply(ctx._h, C.c_int(map_), C.c_int(B), _ptr(X), _ptr(Y), C.c_int(reps), C.byref(ms)),
+           "dmfe_dense_apply")
+    return Y, ms.value
+
+
 def copy_probe(nbytes, reps=10):
     """Average ms of a plain HBM copy of nbytes (counter calibration)."""
     ms = C.c_double()

@@ -105,6 +105,7 @@ struct dmfe_ctx {
     int N = 0, m = 0;
     long long P = 0;
     int split = 2;
+    bool dense_mfma = false;  // dense (B-spline) grid: A1/A2 contracted on MFMA, weights entry-major
     // tables
     double *theta = nullptr, *integ = nullptr, *phi2 = nullptr;
     int* inds[3] = {nullptr, nullptr, nullptr};
@@ -244,6 +245,12 @@ int run_rhs_rows(dmfe_ctx* c, double* fields) {
     a.fields = fields;
     a.fbuf = c->fcur;
     prof_begin(c, kKindRhs);
+    if (c->dense_mfma) {
+        // A1/A2 of a dense grid: (P x N) . (N x 2) on MFMA against the last row (QK, QR)
+        const double* last = reinterpret_cast<const double*>(c->H.V + (size_t)(c->T1 - 1) * c->N);
+        for (int f = 0; f < 2; ++f)
+            launch_dense_apply(c->w[f], last, reinterpret_cast<double*>(c->W.fA[f]), c->P, c->N, 2, 2, c->stream);
+    }
     launch_rhs_rows(c->tables(), c->H, c->W, c->model, a, c->stream);
     prof_end(c, kKindRhs);
     return launch_check("rhs_rows_kernel");
@@ -474,6 +481,7 @@ void free_ctx(dmfe_ctx* c) {
     hipFree(c->W.posB1); hipFree(c->W.posB2); hipFree(c->W.saveB1); hipFree(c->W.saveB2);
     hipFree(c->W.Floor); hipFree(c->W.diff); hipFree(c->W.rInt); hipFree(c->W.partial);
     hipFree(c->W.qkb1c0); hipFree(c->W.hK); hipFree(c->W.hR); hipFree(c->W.scal);
+    hipFree(c->W.fA[0]); hipFree(c->W.fA[1]);
     hipFree(c->fields);
     if (c->pinned) hipHostFree(c->pinned);
     for (auto& v : c->ev)
@@ -528,6 +536,7 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
     c->model.p = prm->p; c->model.p2 = prm->p2;
     c->N = N; c->m = m; c->P = P;
     c->split = (N >= 256) ? std::max(1, N / 256) : 1;
+    c->dense_mfma = (m == N) && !prm->log_response;
     c->specRad = 4 * std::sqrt(host_fl(*prm, 1, 2));
     auto fail = [&](int code, const std::string& msg) { free_ctx(c); return set_error(code, msg); };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(DMFE_ERR_HIP, "stream create failed");
@@ -545,6 +554,9 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
          hipMalloc(&c->W.qkb1c0, bN) == hipSuccess && hipMalloc(&c->W.hK, bN * kMaxStages) == hipSuccess &&
          hipMalloc(&c->W.hR, bN * kMaxStages) == hipSuccess && hipMalloc(&c->W.scal, 8 * sizeof(double)) == hipSuccess &&
          hipHostMalloc((void**)&c->pinned, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    if (ok && c->dense_mfma)
+        ok = hipMalloc(&c->W.fA[0], (size_t)P * sizeof(double2)) == hipSuccess &&
+             hipMalloc(&c->W.fA[1], (size_t)P * sizeof(double2)) == hipSuccess;
     if (!ok) return fail(DMFE_ERR_NOMEM, "dmfe_ctx_create: device allocation failed");
     // uploads: weights transposed [P][m] -> [m][P], stencil starts to int32
     std::vector<double> wt((size_t)P * m);
@@ -555,12 +567,14 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
     for (int f = 0; up && f < 3; ++f) {
         // A1/A2 are read one target per lane: transposed [m][P] so a wavefront's taps are
         // contiguous; B2 is read one tap per lane: kept entry-major [P][m] as on disk.
+        // Dense grids contract A1/A2 on MFMA from the entry-major rows.
+        const bool transpose = f < 2 && !c->dense_mfma;
         for (long long k = 0; k < P; ++k) {
             it[(size_t)k] = g->inds[f][k] < 0 ? 0 : (int)g->inds[f][k];
-            if (f < 2)
+            if (transpose)
                 for (int d = 0; d < m; ++d) wt[(size_t)d * P + k] = g->weights[f][(size_t)k * m + d];
         }
-        const double* src = (f < 2) ? wt.data() : g->weights[f];
+        const double* src = transpose ? wt.data() : g->weights[f];
         up = hipMemcpy(c->inds[f], it.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(c->w[f], src, bP * (size_t)m, hipMemcpyHostToDevice) == hipSuccess;
     }
@@ -853,6 +867,42 @@ extern "C" int dmfe_time_rhs(dmfe_ctx* c, int reps, double* ms_out) {
     c->profiling = was;
     if (rc) return rc;
     if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("dmfe_time_rhs: ") + hipGetErrorString(e));
+    if (ms_out) *ms_out = ms / reps;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_dense_apply(dmfe_ctx* c, int map, int B, const double* X, double* Y, int reps, double* ms_out) {
+    if (!c || !X || !Y || B < 1 || B > 16 || reps < 1 || map < 0 || map > 2)
+        return set_error(DMFE_ERR_ARG, "dmfe_dense_apply: bad argument (map 0..2, 1 <= B <= 16, reps >= 1)");
+    if (c->m != c->N) return set_error(DMFE_ERR_STATE, "dmfe_dense_apply: the grid is not dense (m != N)");
+    if (map < 2 && !c->dense_mfma)
+        return set_error(DMFE_ERR_STATE, "dmfe_dense_apply: A1/A2 tables are stencil-major in log-response mode");
+    const size_t bx = (size_t)c->N * B * sizeof(double), by = (size_t)c->P * B * sizeof(double);
+    double *dx = nullptr, *dy = nullptr;
+    HIP_TRY(hipMalloc(&dx, bx));
+    if (hipMalloc(&dy, by) != hipSuccess) {
+        (void)hipFree(dx);
+        return set_error(DMFE_ERR_NOMEM, "dmfe_dense_apply: output allocation failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemcpyAsync(dx, X, bx, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        launch_dense_apply(c->w[map], dx, dy, c->P, c->N, B, B, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(Y, dy, by, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("dmfe_dense_apply: ") + hipGetErrorString(e));
     if (ms_out) *ms_out = ms / reps;
     return DMFE_OK;
 }

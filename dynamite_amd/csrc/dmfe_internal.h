@@ -65,6 +65,7 @@ struct Work {
     double* hK;       // [kMaxStages][N]
     double* hR;       // [kMaxStages][N]
     double* scal;     // [0] dr (drstep2), [1] error norm, [2] rstep / energy scratch
+    double2* fA[2];   // dense grids: A1, A2 fields (QK, QR) of the current RHS [P], or null
 };
 
 enum SearchMode { kSearchB = 0, kSearchI = 1, kSearchSame = 2, kSearchCopy = 3 };
@@ -114,6 +115,9 @@ void launch_stage(const Tables& T, const History& H, const Work& W, const Model&
                   const FinishArgs* fa, hipStream_t s);
 void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
                      hipStream_t s);
+// Y[P][ldy] (columns 0..B-1) = W[P][K] . X[K][B] in FP64 on MFMA; B <= 16
+void launch_dense_apply(const double* W, const double* X, double* Y, long long P, int K, int B, int ldy,
+                        hipStream_t s);
 // rstep (what = 0) or energy at time t (what = 1) on history row `row`, into W.scal[2]
 void launch_row_scalar(const Tables& T, const History& H, const Work& W, const Model& M, int row, double t,
                        int what, hipStream_t s);

@@ -553,8 +553,14 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
             qkA1 = qkA2 = qkB1 = qkB2 = c00;
             qrA1 = qrA2 = qrB1 = qrB2 = c01;
         } else {
-            const double2 a1 = stencil_last<LOGR>(T.w[0], P, k, m, T.inds[0][k], vlast);
-            const double2 a2 = stencil_last<LOGR>(T.w[1], P, k, m, T.inds[1][k], vlast);
+            double2 a1, a2;
+            if (!LOGR && W.fA[0]) {  // dense grid: A1/A2 contracted beforehand on MFMA
+                a1 = W.fA[0][k];
+                a2 = W.fA[1][k];
+            } else {
+                a1 = stencil_last<LOGR>(T.w[0], P, k, m, T.inds[0][k], vlast);
+                a2 = stencil_last<LOGR>(T.w[1], P, k, m, T.inds[1][k], vlast);
+            }
             qkA1 = a1.x; qrA1 = a1.y; qkA2 = a2.x; qrA2 = a2.y;
             const double2 y0 = VF0[i], y1 = VF1[i], dy1 = DF1[i];
             if (b1int) {
@@ -1061,6 +1067,78 @@ __global__ __launch_bounds__(kStageThreads) void batch_obs_kernel(const InstDesc
     }
 }
 
+// ---- dense (B-spline) interpolation as an FP64 MFMA contraction --------------------
+// Y = W . X with W [P][K] (one dense stencil row per target, entry-major as on disk),
+// X [K][B] (B <= 16 right-hand sides, e.g. the last row's QK and QR) and Y [P][ldy]
+// (index_vec.cpp:15-61 with m = N).  A block of 4 waves owns 64 target rows; K is
+// streamed through LDS in chunks of 64 columns with 16-byte coalesced loads (every
+// row chunk is 512 contiguous bytes), and each wave issues v_mfma_f64_16x16x4f64 on
+// its 16 rows: A[i][k] sits in lane i + 16k, B[k][j] in lane j + 16k, and D[i][j]
+// comes back in lane j + 16 (i % 4), register i / 4.  Columns j >= B are zero.
+constexpr int kDenseRows = 64;
+constexpr int kDenseKC = 32;
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ inline void dense_load_chunk(const double* __restrict__ W, long long P, int K, long long r0, int k0,
+                                        int tid, double2 (&v)[kDenseRows * kDenseKC / 512]) {
+    constexpr int kPairs = kDenseKC / 2;  // column pairs per row chunk
+    const bool full = k0 + kDenseKC <= K && r0 + kDenseRows <= P && (K & 1) == 0;
+#pragma unroll
+    for (int q = 0; q < kDenseRows * kDenseKC / 512; ++q) {
+        const int e = tid + 256 * q;  // pair index: row e / kPairs, columns 2 (e % kPairs) + {0, 1}
+        const int row = e / kPairs, c2 = (e % kPairs) * 2;
+        const long long gr = r0 + row;
+        if (full) {
+            v[q] = *reinterpret_cast<const double2*>(W + gr * K + k0 + c2);
+        } else {
+            v[q].x = (gr < P && k0 + c2 < K) ? W[gr * K + k0 + c2] : 0.0;
+            v[q].y = (gr < P && k0 + c2 + 1 < K) ? W[gr * K + k0 + c2 + 1] : 0.0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void dense_apply_kernel(const double* __restrict__ W, const double* __restrict__ X,
+                                                          double* __restrict__ Y, long long P, int K, int B,
+                                                          int ldy) {
+    __shared__ double ws[kDenseRows][kDenseKC + 1];  // odd row stride: column reads spread over banks
+    __shared__ double xs[kDenseKC][16];
+    constexpr int kPairs = kDenseKC / 2;
+    constexpr int NQ = kDenseRows * kDenseKC / 512;  // 16-byte loads per thread per chunk
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long r0 = (long long)blockIdx.x * kDenseRows;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    double2 pre[NQ];
+    dense_load_chunk(W, P, K, r0, 0, tid, pre);
+    for (int k0 = 0; k0 < K; k0 += kDenseKC) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q;
+            ws[e / kPairs][(e % kPairs) * 2] = pre[q].x;
+            ws[e / kPairs][(e % kPairs) * 2 + 1] = pre[q].y;
+        }
+        for (int e = tid; e < kDenseKC * 16; e += 256) {
+            const int kk = e >> 4, j = e & 15;
+            xs[kk][j] = (k0 + kk < K && j < B) ? X[(long long)(k0 + kk) * B + j] : 0.0;
+        }
+        __syncthreads();
+        // the next chunk's loads are in flight while this chunk is contracted
+        if (k0 + kDenseKC < K) dense_load_chunk(W, P, K, r0, k0 + kDenseKC, tid, pre);
+#pragma unroll
+        for (int ks = 0; ks < kDenseKC; ks += 4) {
+            const double a = ws[wave * 16 + (lane & 15)][ks + (lane >> 4)];
+            const double b = xs[ks + (lane >> 4)][lane & 15];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int j = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const long long gr = r0 + wave * 16 + (lane >> 4) + 4 * r;
+        if (gr < P && j < B) Y[gr * ldy + j] = acc[r];
+    }
+}
+
 // Plain streaming copy, used to calibrate HBM byte counters (FETCH_SIZE / WRITE_SIZE).
 __global__ __launch_bounds__(256) void hbm_copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
                                                        long long n) {
@@ -1069,6 +1147,12 @@ __global__ __launch_bounds__(256) void hbm_copy_kernel(const double2* __restrict
 }
 
 }  // namespace
+
+void launch_dense_apply(const double* W, const double* X, double* Y, long long P, int K, int B, int ldy,
+                        hipStream_t s) {
+    const long long nb = (P + kDenseRows - 1) / kDenseRows;
+    hipLaunchKernelGGL(dense_apply_kernel, dim3((unsigned)nb), dim3(256), 0, s, W, X, Y, P, K, B, ldy);
+}
 
 void launch_hbm_copy(const double2* src, double2* dst, long long n, hipStream_t s) {
     hipLaunchKernelGGL(hbm_copy_kernel, dim3(2048), dim3(256), 0, s, src, dst, n);

@@ -209,6 +209,14 @@ int dmfe_sparsify(dmfe_ctx* ctx, double threshold);
 int dmfe_profile(dmfe_ctx* ctx, int enable);
 int dmfe_profile_read(dmfe_ctx* ctx, int kind, double* total_ms, int64_t* launches);
 
+/* Dense (B-spline, m = N) stencil table `map` (0 A1, 1 A2, 2 B2) applied to B <= 16
+ * right-hand sides as one FP64 MFMA contraction: Y [N*N][B] = W [N*N][N] . X [N][B]
+ * (row-major host buffers).  This is indexVecLN3 with m = N (index_vec.cpp:15-61)
+ * for B vectors at once.  The contraction runs `reps` times between two HIP events;
+ * *ms_out (may be NULL) = average device time per contraction, host copies excluded.
+ * In log-response mode only map 2 is available (A1/A2 keep the stencil layout). */
+int dmfe_dense_apply(dmfe_ctx* ctx, int map, int B, const double* X, double* Y, int reps, double* ms_out);
+
 /* Measurement utility (no counterpart in the reference): `reps` back-to-back
  * launches of the RHS kernel (rhs_rows_kernel) on the current interpolation state,
  * timed with two HIP events on the context's stream; the state is not changed.

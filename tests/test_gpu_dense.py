@@ -1,0 +1,87 @@
+"""Dense B-spline grids (m = N): the FP64 MFMA contraction and the step path on it.
+
+The contraction Y = W . X (dmfe_dense_apply) is checked against numpy's float64
+product with the componentwise rounding bound K * eps * (|W| . |X|) scaled by 4, on
+shapes that exercise partial row blocks (P not a multiple of 64), a partial K chunk,
+odd K (the unaligned load path) and 1..16 right-hand sides.  The interpolated fields,
+the RHS and a quench on a dense grid are then compared with the oracle (which reads
+the intended full-row stencil; the reference's -1 start is a documented deviation).
+"""
+import numpy as np
+import pytest
+
+import dynamite_amd as dm
+from oracle_lib import Oracle, synthetic_history
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = {}
+
+
+def bgrid(N, Tmax=1e2, order=5):
+    key = (N, Tmax, order)
+    if key not in GRIDS:
+        GRIDS[key] = dm.generate_grid(N, Tmax, 3, dm.INTERP_BSPLINE, order, 0)
+    return GRIDS[key]
+
+
+def params(log_resp=0):
+    return dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.8, log_response=log_resp)
+
+
+@pytest.mark.parametrize("N,B", [(48, 4), (40, 1), (37, 3), (64, 16), (130, 2)])
+def test_dense_contraction_matches_float64_product(N, B):
+    g = bgrid(N)
+    assert g.m == N
+    ctx = dm.Context(g, params(), rows_hint=8)
+    rng = np.random.default_rng(20261203 + N)
+    X = rng.uniform(-1.0, 1.0, size=(N, B))
+    for map_ in (0, 1, 2):
+        print(f"dense N={N} B={B} map={map_}", flush=True)
+        W = g.weights[map_].reshape(N * N, N)
+        Y, ms = dm.dense_apply(ctx, X, map_)
+        ref = W @ X
+        bound = 4 * N * 2.2e-16 * (np.abs(W) @ np.abs(X))
+        assert np.all(np.abs(Y - ref) <= bound + 1e-300), np.max(np.abs(Y - ref) / (bound + 1e-300))
+        assert ms > 0
+    ctx.close()
+
+
+@pytest.mark.parametrize("log_resp", [0, 1])
+def test_dense_fields_and_rhs_match_oracle(log_resp):
+    N = 48
+    g = bgrid(N)
+    st = synthetic_history(g.theta)
+    ctx = dm.Context(g, params(log_resp))
+    orc = Oracle(g, p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.8, log_resp=log_resp)
+    ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
+    orc.set_history(st["t1grid"], st["dtratio"], st["rvec"], st["drvec"], st["QK"], st["QR"], st["dQK"], st["dQR"])
+    ctx.interpolate(dm.POS_BSEARCH)
+    orc.interpolate(0)
+    fg, fo = ctx.fields(), orc.fields()
+    errs = {k: np.abs(fg[k] - fo[k]).max() / np.abs(fo[k]).max() for k in dm.FIELD_NAMES + ("rInt",)}
+    tol = 1e-12 if not log_resp else 1e-9
+    assert max(errs.values()) <= tol, {k: f"{v:.2e}" for k, v in errs.items()}
+    hK, hR = ctx.rhs()
+    oK, oR = orc.rhs()
+    assert np.abs(hK - oK).max() <= 1e-12 * np.abs(oK).max()
+    assert np.abs(hR - oR).max() <= 1e-12 * np.abs(oR).max()
+    ctx.close()
+
+
+def test_dense_quench_matches_oracle():
+    N = 40
+    g = bgrid(N)
+    ctx = dm.Context(g, params())
+    orc = Oracle(g, p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.8)
+    ctx.init_quench()
+    orc.init_quench()
+    for _ in range(60):
+        tg = ctx.loop_iteration()
+        to = orc.loop_iteration()
+        assert tg == to
+    sg, so = ctx.get_state(), orc.state()
+    assert np.array_equal(sg["t1grid"], so["t1grid"])
+    for key in ("QK", "QR"):
+        assert np.abs(sg[key] - so[key]).max() <= 1e-11 * np.abs(so[key]).max(), key
+    ctx.close()

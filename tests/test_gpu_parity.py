@@ -165,3 +165,25 @@ def test_sparsify_matches_oracle():
     sg, so = ctx.get_state(), orc.state()
     assert np.array_equal(sg["t1grid"], so["t1grid"])
     rel_close(sg["QK"], so["QK"], 1e-11)
+
+
+def test_full_size_L512_steps_match_oracle():
+    """The bench grid family at its full size (L=512, poly 9, Tmax 1e5): eight main-loop
+    iterations from the quench, GPU against the oracle."""
+    g = dm.generate_grid(512, 1e5, 5, dm.INTERP_POLY, 9, 0)
+    ctx, orc = pair(g, Gamma=0.6)
+    ctx.init_quench()
+    orc.init_quench()
+    for _ in range(8):
+        tg = ctx.loop_iteration()
+        to = orc.loop_iteration()
+        assert tg == to
+    sg, so = ctx.get_state(), orc.state()
+    assert np.array_equal(sg["t1grid"], so["t1grid"])
+    for key in ("QK", "QR", "dQK", "dQR", "rvec", "drvec"):
+        rel_close(sg[key], so[key], 1e-11)
+    b1, b2 = ctx.positions()
+    o1, o2 = orc.positions()
+    assert np.array_equal(b1, o1) and np.array_equal(b2, o2)
+    assert abs(ctx.observables(tg)[0] - orc.energy(to)) <= 1e-11 * abs(orc.energy(to))
+    ctx.close()
