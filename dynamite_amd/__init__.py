@@ -280,6 +280,34 @@ class Context:
     def sparsify(self, threshold):
         _check(lib().dmfe_sparsify(self._h, C.c_double(threshold)), "dmfe_sparsify")
 
+    # -- parameter sensitivities (forward mode) ------------------------------------
+    def tangent_init(self, dGamma, dLambda):
+        g = np.ascontiguousarray(dGamma, np.float64).ravel()
+        l = np.ascontiguousarray(dLambda, np.float64).ravel()
+        _check(lib().dmfe_tangent_init(self._h, C.c_int(len(g)), _ptr(g), _ptr(l)), "dmfe_tangent_init")
+        self._tanK = len(g)
+
+    def tangent_observables(self, t):
+        dE, dQ = np.zeros(self._tanK), np.zeros(self._tanK)
+        _check(lib().dmfe_tangent_observables(self._h, C.c_double(t), _ptr(dE), _ptr(dQ)), "dmfe_tangent_observables")
+        return dE, dQ
+
+    def tangent_state(self, k):
+        T1, N = self.T1, self.N
+        dQK, dQR, dr = np.zeros(T1 * N), np.zeros(T1 * N), np.zeros(T1)
+        _check(lib().dmfe_tangent_state(self._h, C.c_int(k), _ptr(dQK), _ptr(dQR), _ptr(dr)), "dmfe_tangent_state")
+        return dict(QK=dQK, QR=dQR, rvec=dr)
+
+    def gradients(self, tmax, max_loop):
+        """[dE/dGamma, dE/dlambda, dQK0/dGamma, dQK0/dlambda] of a quench run to tmax / max_loop,
+        plus the final proposed time and the number of steps."""
+        grad = np.zeros(4)
+        t, n = C.c_double(), C.c_int64()
+        _check(lib().dmfe_gradients(self._h, C.c_double(tmax), C.c_int64(max_loop), _ptr(grad), C.byref(t),
+                                    C.byref(n)), "dmfe_gradients")
+        self._tanK = 2
+        return grad, t.value, n.value
+
     def synchronize(self):
         _check(lib().dmfe_synchronize(self._h), "dmfe_synchronize")
 

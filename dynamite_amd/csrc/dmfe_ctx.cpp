@@ -106,6 +106,11 @@ struct dmfe_ctx {
     long long P = 0;
     int split = 2;
     bool dense_mfma = false;  // dense (B-spline) grid: A1/A2 contracted on MFMA, weights entry-major
+    // forward-mode sensitivities (dmfe_tangent_init): tangent histories and work
+    int tanK = 0;
+    History tanH[kMaxTan] = {};
+    double tanG[kMaxTan] = {}, tanL[kMaxTan] = {};
+    double* tanWork = nullptr;
     // tables
     double *theta = nullptr, *integ = nullptr, *phi2 = nullptr;
     int* inds[3] = {nullptr, nullptr, nullptr};
@@ -128,6 +133,30 @@ struct dmfe_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev[kKinds];  // start/stop pairs
     size_t ev_used[kKinds] = {0, 0, 0, 0};
+
+    TanArgs tan_args() const {
+        TanArgs X{};
+        X.K = tanK;
+        for (int k = 0; k < tanK; ++k) {
+            X.dG[k] = tanG[k];
+            X.dL[k] = tanL[k];
+            X.V[k] = tanH[k].V;
+            X.D[k] = tanH[k].D;
+            X.r[k] = tanH[k].r;
+            X.dr[k] = tanH[k].dr;
+        }
+        // tanWork: partial [K][N][split][8] | qkb1c0 [K][N] | hK, hR [K][10][N] | scal [K][8]
+        double* p = tanWork;
+        X.partial = p; p += (size_t)tanK * N * split * kSums;
+        X.qkb1c0 = p; p += (size_t)tanK * N;
+        X.hK = p; p += (size_t)tanK * kMaxStages * N;
+        X.hR = p; p += (size_t)tanK * kMaxStages * N;
+        X.scal = p;
+        return X;
+    }
+    size_t tan_work_doubles() const {
+        return (size_t)tanK * ((size_t)N * split * kSums + N + 2 * (size_t)kMaxStages * N + 8);
+    }
 
     Tables tables() const {
         Tables T{};
@@ -174,10 +203,30 @@ int ensure_rows(dmfe_ctx* c, int64_t rows) {
         HIP_TRY(hipMemcpyAsync(h.dr, c->H.dr, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
+    for (int k = 0; k < c->tanK; ++k) {
+        History th{};
+        rc = alloc_history(th, nc, c->N);
+        if (rc) { free_history(h); return rc; }
+        const size_t n = (size_t)c->T1;
+        HIP_TRY(hipMemcpyAsync(th.V, c->tanH[k].V, n * c->N * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(th.D, c->tanH[k].D, n * c->N * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(th.r, c->tanH[k].r, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(th.dr, c->tanH[k].dr, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        free_history(c->tanH[k]);
+        c->tanH[k] = th;
+    }
     free_history(c->H);
     c->H = h;
     c->cap = nc;
     return DMFE_OK;
+}
+
+void tangent_free(dmfe_ctx* c) {
+    for (int k = 0; k < kMaxTan; ++k) free_history(c->tanH[k]);
+    if (c->tanWork) (void)hipFree(c->tanWork);
+    c->tanWork = nullptr;
+    c->tanK = 0;
 }
 
 void prof_begin(dmfe_ctx* c, int kind) {
@@ -251,7 +300,10 @@ int run_rhs_rows(dmfe_ctx* c, double* fields) {
         for (int f = 0; f < 2; ++f)
             launch_dense_apply(c->w[f], last, reinterpret_cast<double*>(c->W.fA[f]), c->P, c->N, 2, 2, c->stream);
     }
-    launch_rhs_rows(c->tables(), c->H, c->W, c->model, a, c->stream);
+    if (c->tanK && !fields)  // the same RHS plus the tangent quadratures
+        launch_tan_rhs(c->tables(), c->H, c->W, c->model, a, c->tan_args(), c->stream);
+    else
+        launch_rhs_rows(c->tables(), c->H, c->W, c->model, a, c->stream);
     prof_end(c, kKindRhs);
     return launch_check("rhs_rows_kernel");
 }
@@ -358,6 +410,14 @@ int rk_step(dmfe_ctx* c, double* err_out) {
             }
             f.gt_err = std::fabs(gt[S] - gte);
         }
+        if (c->tanK) {
+            // before the primal finish of this stage, which replaces the RHS's last row
+            FinishArgs tf = f;
+            tf.fbuf = c->fcur;
+            launch_tan_finish(c->tables(), c->H, c->W, c->model, tf, c->tan_args(), c->stream);
+            rc = launch_check("tan_finish_kernel");
+            if (rc) return rc;
+        }
         have_pending = true;
     }
     // final interpolation: RK54 reuses the positions, SSPRK104 searches (:121, :229)
@@ -371,6 +431,7 @@ int rk_step(dmfe_ctx* c, double* err_out) {
 }
 
 int sparsify(dmfe_ctx* c, double threshold) {
+    if (c->tanK) return set_error(DMFE_ERR_STATE, "sparsify: not available while sensitivities are carried");
     const int64_t T1 = c->T1;
     std::vector<int> keep;
     keep.reserve((size_t)T1);
@@ -418,6 +479,10 @@ int sparsify(dmfe_ctx* c, double threshold) {
 
 // one iteration of runSimulation's while-loop body (simulation_runner.cpp:128-278, CPU-only branches)
 int loop_iteration(dmfe_ctx* c, double* t_out) {
+    const int64_t every_ = c->params.maint_every > 0 ? c->params.maint_every : 100000;
+    if (c->tanK && (c->loop + 1) % every_ == 0)
+        return set_error(DMFE_ERR_STATE, "sensitivities: the run would cross a sparsification point (loop " +
+                                             std::to_string(c->loop + 1) + ")");
     const double t = c->t1[c->T1 - 1] + c->dt;
     c->delta_old = c->delta;
     double err = 0.0;
@@ -482,6 +547,7 @@ void free_ctx(dmfe_ctx* c) {
     hipFree(c->W.Floor); hipFree(c->W.diff); hipFree(c->W.rInt); hipFree(c->W.partial);
     hipFree(c->W.qkb1c0); hipFree(c->W.hK); hipFree(c->W.hR); hipFree(c->W.scal);
     hipFree(c->W.fA[0]); hipFree(c->W.fA[1]);
+    tangent_free(c);
     hipFree(c->fields);
     if (c->pinned) hipHostFree(c->pinned);
     for (auto& v : c->ev)
@@ -598,6 +664,7 @@ extern "C" int dmfe_ctx_destroy(dmfe_ctx* c) {
 
 extern "C" int dmfe_init_quench(dmfe_ctx* c) {
     if (!c) return set_error(DMFE_ERR_ARG, "null context");
+    tangent_free(c);
     const int N = c->N;
     int rc = ensure_rows(c, 1);
     if (rc) return rc;
@@ -638,6 +705,7 @@ extern "C" int dmfe_get_T1(const dmfe_ctx* c, int64_t* T1) {
 extern "C" int dmfe_set_state(dmfe_ctx* c, int64_t T1, const double* t1grid, const double* QK, const double* QR,
                               const double* dQK, const double* dQR, const double* rvec, const double* drvec,
                               const double* dtratio) {
+    if (c) tangent_free(c);
     if (!c || T1 < 1 || !t1grid || !QK || !QR || !dQK || !dQR || !rvec || !drvec)
         return set_error(DMFE_ERR_ARG, "dmfe_set_state: null argument or T1 < 1");
     for (int64_t i = 1; i < T1; ++i)
@@ -844,6 +912,90 @@ extern "C" int dmfe_profile_read(dmfe_ctx* c, int kind, double* total_ms, int64_
     }
     *total_ms = tot;
     *launches = (int64_t)(n / 2);
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_tangent_init(dmfe_ctx* c, int K, const double* dGamma, const double* dLambda) {
+    if (!c || K < 1 || K > kMaxTan || !dGamma || !dLambda)
+        return set_error(DMFE_ERR_ARG, "dmfe_tangent_init: bad argument (1 <= K <= 2)");
+    if (c->T1 != 1) return set_error(DMFE_ERR_STATE, "dmfe_tangent_init: call right after dmfe_init_quench");
+    if (c->params.log_response)
+        return set_error(DMFE_ERR_STATE, "dmfe_tangent_init: the log-response interpolation is not differentiated");
+    tangent_free(c);
+    for (int k = 0; k < K; ++k) {
+        int rc = alloc_history(c->tanH[k], c->cap, c->N);
+        if (rc) { tangent_free(c); return rc; }
+        c->tanG[k] = dGamma[k];
+        c->tanL[k] = dLambda[k];
+    }
+    c->tanK = K;
+    if (hipMalloc(&c->tanWork, c->tan_work_doubles() * sizeof(double)) != hipSuccess) {
+        tangent_free(c);
+        return set_error(DMFE_ERR_NOMEM, "dmfe_tangent_init: work allocation failed");
+    }
+    HIP_TRY(hipMemsetAsync(c->tanWork, 0, c->tan_work_doubles() * sizeof(double), c->stream));
+    launch_tan_init_n(c->model, c->tan_args(), c->N, c->stream);
+    int rc = launch_check("tan_init_kernel");
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_tangent_observables(dmfe_ctx* c, double t, double* dE, double* dQK0) {
+    if (!c || c->tanK < 1) return set_error(DMFE_ERR_STATE, "dmfe_tangent_observables: no sensitivities carried");
+    const TanArgs X = c->tan_args();
+    launch_tan_energy(c->tables(), c->H, c->model, X, (int)(c->T1 - 1), t, c->stream);
+    int rc = launch_check("tan_energy_kernel");
+    if (rc) return rc;
+    for (int k = 0; k < c->tanK; ++k) {
+        double e = 0.0, q = 0.0;
+        HIP_TRY(hipMemcpyAsync(&e, X.scal + k * 8 + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(&q, &X.V[k][(size_t)(c->T1 - 1) * c->N].x, sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (dE) dE[k] = e;
+        if (dQK0) dQK0[k] = q;
+    }
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_tangent_state(dmfe_ctx* c, int k, double* dQK, double* dQR, double* dr) {
+    if (!c || k < 0 || k >= c->tanK) return set_error(DMFE_ERR_ARG, "dmfe_tangent_state: bad direction");
+    const size_t n = (size_t)c->T1 * c->N;
+    std::vector<double2> v(n);
+    HIP_TRY(hipMemcpyAsync(v.data(), c->tanH[k].V, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    if (dr) HIP_TRY(hipMemcpyAsync(dr, c->tanH[k].r, (size_t)c->T1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; ++i) {
+        if (dQK) dQK[i] = v[i].x;
+        if (dQR) dQR[i] = v[i].y;
+    }
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_gradients(dmfe_ctx* c, double tmax, int64_t max_loop, double* grad, double* t_final,
+                              int64_t* steps) {
+    if (!c || !grad) return set_error(DMFE_ERR_ARG, "dmfe_gradients: null argument");
+    int rc = dmfe_init_quench(c);
+    if (rc) return rc;
+    const double g[2] = {1.0, 0.0}, l[2] = {0.0, 1.0};
+    rc = dmfe_tangent_init(c, 2, g, l);
+    if (rc) return rc;
+    int64_t done = 0;
+    double t = c->t1[c->T1 - 1];
+    const double eps = std::numeric_limits<double>::epsilon();
+    while (t < tmax && c->loop < max_loop &&
+           c->dt >= c->params.delta_t_min - eps * std::max(std::fabs(c->dt), std::fabs(c->params.delta_t_min))) {
+        rc = loop_iteration(c, &t);
+        if (rc) return rc;
+        ++done;
+    }
+    double dE[2], dQ[2];
+    rc = dmfe_tangent_observables(c, t, dE, dQ);
+    if (rc) return rc;
+    grad[0] = dE[0]; grad[1] = dE[1]; grad[2] = dQ[0]; grad[3] = dQ[1];
+    if (t_final) *t_final = t;
+    if (steps) *steps = done;
     return DMFE_OK;
 }
 

@@ -110,6 +110,31 @@ struct FinishArgs {
     int fbuf;                  // Floor/diff buffer of the interpolation this RHS used
 };
 
+// Forward-mode sensitivities: K <= kMaxTan directions, direction k perturbs (Gamma, lambda)
+// by (dG[k], dL[k]).  Each direction carries a tangent history (V, D, r, dr; t1grid and
+// delta_t_ratio are the primal's, the time grid being frozen) and its own work arrays.
+constexpr int kMaxTan = 2;
+struct TanArgs {
+    int K;
+    double dG[kMaxTan], dL[kMaxTan];
+    double2* V[kMaxTan];   // [cap][N] tangent (QK, QR)
+    double2* D[kMaxTan];   // [cap][N] tangent (dQK, dQR)
+    double* r[kMaxTan];    // [cap]
+    double* dr[kMaxTan];   // [cap]
+    double* partial;       // [K][N][split][kSums] tangent quadratures
+    double* qkb1c0;        // [K][N]
+    double* hK;            // [K][kMaxStages][N]
+    double* hR;            // [K][kMaxStages][N]
+    double* scal;          // [K][8]: 0 tangent dr of the step, 2 tangent energy
+};
+void launch_tan_init_n(const Model& M, const TanArgs& X, int N, hipStream_t s);
+void launch_tan_rhs(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
+                    const TanArgs& X, hipStream_t s);
+void launch_tan_finish(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& f,
+                       const TanArgs& X, hipStream_t s);
+void launch_tan_energy(const Tables& T, const History& H, const Model& M, const TanArgs& X, int row, double t,
+                       hipStream_t s);
+
 // finish of the previous stage (fa) and/or the next interpolation's searches (sa) in one launch
 void launch_stage(const Tables& T, const History& H, const Work& W, const Model& M, const SearchArgs* sa,
                   const FinishArgs* fa, hipStream_t s);

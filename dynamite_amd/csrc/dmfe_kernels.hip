@@ -364,6 +364,39 @@ __device__ inline int xcd_remap(int b, int nb) {
     return (b % X) * (nb / X) + b / X;
 }
 
+// ---- forward-mode sensitivities with respect to Gamma and lambda -----------------------
+// The time grid (dt sequence, positions, stencil starts, Hermite abscissae) is frozen,
+// so the step map is a smooth function of the history values and of (Gamma, lambda).
+// Its tangent: interpolants are linear in the history, hence their tangents are the same
+// stencils / Hermite forms on the tangent history; f', f'' and f''' follow the chain rule
+// in q and their explicit lambda dependence (math_ops.cpp:27-37); Gamma enters through
+// 2 Gamma qR in QKstep (time_steps.cpp:43) and through rstep (:91).  Lane per target,
+// written for clarity: the tangents cost about two primal RHS evaluations per stage.
+__device__ inline double fl1l(const Model& M, double q) {  // derivative of f' in lambda
+    return M.p * pow_int(q, M.p - 1) - M.p2 * pow_int(q, M.p2 - 1);
+}
+__device__ inline double fl2l(const Model& M, double q) {  // derivative of f'' in lambda
+    return M.p * (M.p - 1) * pow_int(q, M.p - 2) - M.p2 * (M.p2 - 1) * pow_int(q, M.p2 - 2);
+}
+__device__ inline double fl3l(const Model& M, double q) {  // derivative of f''' in lambda
+    return M.p * (M.p - 1) * (M.p - 2) * pow_int(q, M.p - 3) -
+           M.p2 * (M.p2 - 1) * (M.p2 - 2) * pow_int(q, M.p2 - 3);
+}
+__device__ inline double fl4(const Model& M, double q) {  // fourth derivative of f
+    return M.lambda * M.p * (M.p - 1) * (M.p - 2) * (M.p - 3) * pow_int(q, M.p - 4) +
+           (1 - M.lambda) * M.p2 * (M.p2 - 1) * (M.p2 - 2) * (M.p2 - 3) * pow_int(q, M.p2 - 4);
+}
+
+// history view of tangent direction k (time arrays are the primal's)
+__device__ inline History tan_history(const History& H, const TanArgs& X, int k) {
+    History h = H;
+    h.V = X.V[k];
+    h.D = X.D[k];
+    h.r = X.r[k];
+    h.dr = X.dr[k];
+    return h;
+}
+
 __device__ inline double rint_at(const History& H, const int* Floor, const double* diff, int i, int T1, bool live) {
     if (!live) return H.r[0];
     const size_t f = (size_t)Floor[i], t1len = (size_t)T1;
@@ -488,10 +521,117 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
 //  phase 2 (lane per entry): A1/A2 stencils on the last row (LDS), the B1 time
 //          Hermite (index_vec.cpp:63-136), the self-energies and the weighted
 //          products, then one block reduction.
-template <bool LOGR, int G>
+// B2 of the primal history and of K tangent histories in one pass (linear form): the
+// weights and positions are loaded once per tap and applied to every history; each
+// history's sums and Hermite are the same operations as in b2_groups.
+template <int G, int K>
+__device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& H, const TanArgs& X, const RhsArgs& a,
+                                                int j, int c0, int C, const int* b2ix, const int* b2s,
+                                                const double* b2u, double2* b2, double2* tb2, int cw) {
+    const int N = T.N, m = T.m, T1 = a.T1;
+    constexpr int GPW = kWave / G;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int g = lane / G, d0 = lane % G;
+    constexpr int kWaves = kRowThreads / kWave;
+    const double2* Vh[1 + K];
+    const double2* Dh[1 + K];
+    Vh[0] = H.V;
+    Dh[0] = H.D;
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+        Vh[1 + h] = X.V[h];
+        Dh[1 + h] = X.D[h];
+    }
+    for (int e0 = wid * GPW; e0 < C; e0 += GPW * kWaves) {
+        const int e = e0 + g;
+        const bool ev = e < C;
+        const int ee = ev ? e : 0;
+        const size_t ix = (size_t)b2ix[ee];
+        const int sst = b2s[ee];
+        const bool b2int = ix < (size_t)(T1 - 1);
+        const size_t o0 = (ix - 1) * N + sst, o1 = ix * N + sst, o2 = (b2int ? ix + 1 : ix) * N + sst;
+        const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
+        double acc8[1 + K][8];
+#pragma unroll
+        for (int h = 0; h <= K; ++h)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc8[h][q] = 0.0;
+        if (ev) {
+            for (int dd = d0; dd < m; dd += G) {
+                const double wd = wk[dd];
+#pragma unroll
+                for (int h = 0; h <= K; ++h) {
+                    const double2 v0 = Vh[h][o0 + dd], v1 = Vh[h][o1 + dd], g1 = Dh[h][o1 + dd], g2 = Dh[h][o2 + dd];
+                    acc8[h][0] += wd * v0.x; acc8[h][1] += wd * v0.y;
+                    acc8[h][2] += wd * v1.x; acc8[h][3] += wd * v1.y;
+                    acc8[h][4] += wd * g1.x; acc8[h][5] += wd * g1.y;
+                    acc8[h][6] += wd * g2.x; acc8[h][7] += wd * g2.y;
+                }
+            }
+        }
+        const double u = b2u[ee], u2 = u * u, u3 = u2 * u;
+        const double rr = b2int ? H.rho[ix + 1] : 1.0;
+#pragma unroll
+        for (int h = 0; h <= K; ++h) {
+            const double y0k = group_sum<G>(acc8[h][0]), y0r = group_sum<G>(acc8[h][1]);
+            const double y1k = group_sum<G>(acc8[h][2]), y1r = group_sum<G>(acc8[h][3]);
+            const double e1k = group_sum<G>(acc8[h][4]), e1r = group_sum<G>(acc8[h][5]);
+            const double e2k = group_sum<G>(acc8[h][6]), e2r = group_sum<G>(acc8[h][7]);
+            if (d0 == 0 && ev) {
+                double qk, qr;
+                if (b2int) {
+                    const double h0 = 1 - 3 * u2 + 2 * u3, h1 = u - 2 * u2 + u3, h2 = 3 * u2 - 2 * u3, h3 = -u2 + u3;
+                    qk = h0 * y0k + h1 * e1k + h2 * y1k + h3 * e2k / rr;
+                    qr = h0 * y0r + h1 * e1r + h2 * y1r + h3 * e2r / rr;
+                } else {
+                    qk = (1 - u2) * y0k + u2 * y1k + (u - u2) * e1k;
+                    qr = (1 - u2) * y0r + u2 * y1r + (u - u2) * e1r;
+                }
+                if (h == 0) b2[e] = make_double2(qk, qr);
+                else tb2[(size_t)(h - 1) * cw + e] = make_double2(qk, qr);
+            }
+        }
+    }
+}
+
+#pragma clang fp contract(off)  // the primal part must repeat stencil_last<false> exactly
+// A1 or A2 stencil on the last rows of the primal and of K tangent histories (LDS),
+// weights loaded once per tap; `dense`: entry-major weights (the primal then comes
+// from the MFMA contraction and only the tangents are summed here).
+template <int K>
+__device__ inline void stencil_last_multi(const double* __restrict__ w, long long P, long long k, int m, int s,
+                                          bool dense, const double2* __restrict__ vl, const double2* __restrict__ tvl,
+                                          int N, double2& prim, double2 (&tan)[K]) {
+    double a = 0.0, b = 0.0;
+    double ta[K], tb[K];
+#pragma unroll
+    for (int h = 0; h < K; ++h) ta[h] = tb[h] = 0.0;
+    for (int d = 0; d < m; ++d) {
+        const double wd = dense ? w[k * m + d] : w[(long long)d * P + k];
+        const double2 v = vl[s + d];
+        a = a + wd * v.x;
+        b = b + wd * v.y;
+#pragma unroll
+        for (int h = 0; h < K; ++h) {
+            const double2 tv = tvl[(size_t)h * N + s + d];
+            ta[h] = ta[h] + wd * tv.x;
+            tb[h] = tb[h] + wd * tv.y;
+        }
+    }
+    prim = make_double2(a, b);
+#pragma unroll
+    for (int h = 0; h < K; ++h) tan[h] = make_double2(ta[h], tb[h]);
+}
+#pragma clang fp contract(on)
+
+// K > 0: also the tangent quadratures of K forward-mode directions (linear-response
+// form only): the tangent interpolants are the same stencils on the tangent histories,
+// the primal part is unchanged.
+template <bool LOGR, int G, int K = 0>
 __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
-                              int j, int chunk, double2* dyn /* [N] last row, [cw] (QKB2, QRB2), [cw] u, [cw] ix, s */,
-                              double* red) {
+                              int j, int chunk, double2* dyn /* [N] last row, [cw] (QKB2, QRB2), [cw] u, [cw] ix, s;
+                                                                K > 0: [K N] tangent last rows, [K cw] tangent B2 */,
+                              double* red, const TanArgs* X = nullptr) {
     const int N = T.N, m = T.m;
     const long long P = T.P;
     const int cw = (N + a.split - 1) / a.split;
@@ -505,6 +645,13 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     int* b2s = b2ix + cw;
     const double2* Vl = H.V + (size_t)(T1 - 1) * N;
     for (int l = threadIdx.x; l < N; l += kRowThreads) vlast[l] = Vl[l];
+    double2* tvlast = reinterpret_cast<double2*>(b2s + cw);  // K > 0 only
+    double2* tb2 = tvlast + (size_t)K * N;
+    if constexpr (K > 0) {
+#pragma unroll
+        for (int h = 0; h < K; ++h)
+            for (int l = threadIdx.x; l < N; l += kRowThreads) tvlast[h * N + l] = X->V[h][(size_t)(T1 - 1) * N + l];
+    }
 
     // ---- phase 1: B2 ----------------------------------------------------------------
     const double c00 = H.V[0].x, c01 = H.V[0].y;
@@ -526,7 +673,8 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
         __syncthreads();
         // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
         // instruction touch 64/G entries x a few contiguous cache lines
-        b2_groups<LOGR, G>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2);
+        if constexpr (K == 0) b2_groups<LOGR, G>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2);
+        else b2_groups_multi<G, K>(T, H, *X, a, j, c0, C, b2ix, b2s, b2u, b2, tb2, cw);
     }
     __syncthreads();
 
@@ -545,16 +693,28 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     double acc[kSums];
 #pragma unroll
     for (int q = 0; q < kSums; ++q) acc[q] = 0.0;
+    double tacc[kSums * (K > 0 ? K : 1)];
+#pragma unroll
+    for (int q = 0; q < kSums * (K > 0 ? K : 1); ++q) tacc[q] = 0.0;
 
     for (int i = c0 + threadIdx.x; i < c1; i += kRowThreads) {
         const long long k = (long long)j * N + i;
         double qkA1, qrA1, qkA2, qrA2, qkB1, qrB1, qkB2, qrB2;
+        double2 ta1[K > 0 ? K : 1], ta2[K > 0 ? K : 1];  // tangent A1, A2 (K > 0)
         if (!live) {
             qkA1 = qkA2 = qkB1 = qkB2 = c00;
             qrA1 = qrA2 = qrB1 = qrB2 = c01;
         } else {
             double2 a1, a2;
-            if (!LOGR && W.fA[0]) {  // dense grid: A1/A2 contracted beforehand on MFMA
+            if constexpr (K > 0) {
+                const bool dense = W.fA[0] != nullptr;
+                stencil_last_multi<K>(T.w[0], P, k, m, T.inds[0][k], dense, vlast, tvlast, N, a1, ta1);
+                stencil_last_multi<K>(T.w[1], P, k, m, T.inds[1][k], dense, vlast, tvlast, N, a2, ta2);
+                if (dense) {
+                    a1 = W.fA[0][k];
+                    a2 = W.fA[1][k];
+                }
+            } else if (!LOGR && W.fA[0]) {  // dense grid: A1/A2 contracted beforehand on MFMA
                 a1 = W.fA[0][k];
                 a2 = W.fA[1][k];
             } else {
@@ -596,6 +756,55 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
             for (int q = 0; q < 15; ++q) f[(long long)q * P + k] = v[q];
         }
         if (i == 0) W.qkb1c0[j] = qkB1;
+        if constexpr (K > 0) {
+#pragma unroll
+            for (int h = 0; h < K; ++h) {
+                double t[8];  // tangent qkA1 qrA1 qkA2 qrA2 qkB1 qrB1 qkB2 qrB2
+                if (!live) {
+                    const double2 v0 = X->V[h][0];
+                    t[0] = t[2] = t[4] = t[6] = v0.x;
+                    t[1] = t[3] = t[5] = t[7] = v0.y;
+                } else {
+                    t[0] = ta1[h].x;
+                    t[1] = ta1[h].y;
+                    t[2] = ta2[h].x;
+                    t[3] = ta2[h].y;
+                    const double2* tV = X->V[h];
+                    const double2* tD = X->D[h];
+                    const double2 y0 = tV[(size_t)(F - 1) * N + i], y1 = tV[(size_t)F * N + i];
+                    const double2 dy1 = tD[(size_t)F * N + i], dy2 = tD[(size_t)(b1int ? F + 1 : F) * N + i];
+                    if (b1int) {
+                        t[4] = b1a * y0.x + b1b * y1.x - b1c * dy1.x - b1d * dy2.x / b1rho;
+                        t[5] = b1a * y0.y + b1b * y1.y - b1c * dy1.y - b1d * dy2.y / b1rho;
+                    } else {
+                        t[4] = (1 - in3) * y0.x - (dB + in3) * dy1.x + in3 * y1.x;
+                        t[5] = (1 - in3) * y0.y - (dB + in3) * dy1.y + in3 * y1.y;
+                    }
+                    const double2 tb = tb2[(size_t)h * cw + (i - c0)];
+                    t[6] = tb.x;
+                    t[7] = tb.y;
+                }
+                const double l = X->dL[h];
+                const double dsKA1 = fl2(M, qkA1) * t[0] + l * fl1l(M, qkA1);
+                const double dsRA1 = (fl3(M, qkA1) * t[0] + l * fl2l(M, qkA1)) * qrA1 + fl2(M, qkA1) * t[1];
+                const double dsRA2 = (fl3(M, qkA2) * t[2] + l * fl2l(M, qkA2)) * qrA2 + fl2(M, qkA2) * t[3];
+                const double dsKB1 = fl2(M, qkB1) * t[4] + l * fl1l(M, qkB1);
+                const double dsRB1 = (fl3(M, qkB1) * t[4] + l * fl2l(M, qkB1)) * qrB1 + fl2(M, qkB1) * t[5];
+                const double dsKB2 = fl2(M, qkB2) * t[6] + l * fl1l(M, qkB2);
+                const double dsRB2 = (fl3(M, qkB2) * t[6] + l * fl2l(M, qkB2)) * qrB2 + fl2(M, qkB2) * t[7];
+                const double wt = T.integ[i];
+                double* ac = tacc + kSums * h;
+                ac[0] += wt * (dsRA2 * qkB2 + sRA2 * t[6]);
+                ac[1] += wt * (dsRA1 * qkB1 + sRA1 * t[4]);
+                ac[2] += wt * (dsKA1 * qrB1 + sKA1 * t[5]);
+                ac[3] += wt * (t[3] * sKB2 + qrA2 * dsKB2);
+                ac[4] += wt * (t[1] * sKB1 + qrA1 * dsKB1);
+                ac[5] += wt * (t[0] * sRB1 + qkA1 * dsRB1);
+                ac[6] += wt * (dsRA2 * qrB2 + sRA2 * t[7]);
+                ac[7] += wt * (t[3] * sRB2 + qrA2 * dsRB2);
+                if (i == 0) X->qkb1c0[(size_t)h * N + j] = t[4];
+            }
+        }
         // quadratures of time_steps.cpp:36-45, 97-98 (weights int.dat)
         const double wi = T.integ[i];
         acc[0] += wi * sRA2 * qkB2;  // ConvR(SigmaRA2, QKB2)
@@ -609,6 +818,16 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     }
     const double tot = block_sum8<kRowThreads>(acc, red);
     if (threadIdx.x < kSums) W.partial[((size_t)j * a.split + chunk) * kSums + threadIdx.x] = tot;
+    if constexpr (K > 0) {
+#pragma unroll
+        for (int h = 0; h < K; ++h) {
+            double v8[kSums];
+#pragma unroll
+            for (int q = 0; q < kSums; ++q) v8[q] = tacc[kSums * h + q];
+            const double tt = block_sum8<kRowThreads>(v8, red);
+            if (threadIdx.x < kSums) X->partial[(((size_t)h * N + j) * a.split + chunk) * kSums + threadIdx.x] = tt;
+        }
+    }
 }
 
 template <bool LOGR, int G>
@@ -617,6 +836,16 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History
     __shared__ double red[kSums * (kRowThreads / kWave)];
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     rhs_rows_body<LOGR, G>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red);
+}
+
+// the RHS with the tangent quadratures of K sensitivity directions (linear-response form)
+template <int G, int K>
+__global__ __launch_bounds__(kRowThreads) void rhs_rows_tan_kernel(Tables T, History H, Work W, Model M, RhsArgs a,
+                                                                   TanArgs X) {
+    extern __shared__ double2 dyn[];
+    __shared__ double red[kSums * (kRowThreads / kWave)];
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    rhs_rows_body<false, G, K>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red, &X);
 }
 
 // ---- finish: RHS epilogue + RK stage -------------------------------------------------
@@ -1139,6 +1368,224 @@ __global__ __launch_bounds__(256) void dense_apply_kernel(const double* __restri
     }
 }
 
+__global__ void tan_init_kernel(Model M, TanArgs X, int N) {
+    const int k = blockIdx.x;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        X.V[k][j] = make_double2(0.0, 0.0);
+        X.D[k][j] = make_double2(0.0, 0.0);
+    }
+    if (threadIdx.x == 0) {
+        // r0 = Gamma + f'(1)/T0 and dr0 = rstep() at t = 0 (initialization.cpp:219-222)
+        const double d = X.dG[k] + X.dL[k] * fl1l(M, 1.0) / M.T0;
+        X.r[k][0] = d;
+        X.dr[k][0] = d;
+    }
+}
+
+// Tangent of finish_block for stage a.stage: tangent RHS (hK, hR), tangent drstep2 at
+// stage 0, tangent stage combination and row write, tangent rstep on the new row.  It
+// runs before the primal finish of the same stage (which overwrites the RHS's last row
+// at stages >= 1) and recomputes the primal quantities it needs.
+template <int EPT, int K>
+__global__ __launch_bounds__(kStageThreads) void tan_finish_kernel(Tables T, History H, Work W, Model M, FinishArgs a,
+                                                                   TanArgs X) {
+    __shared__ double red[6 * K * (kStageThreads / kWave)];
+    const int N = T.N, n = a.stage;
+    const double t = a.t_rhs;
+    const double2* last = H.V + (size_t)a.lr * N;
+    const bool live = t > 0;
+    const int* Fl = W.Floor + (size_t)a.fbuf * N;
+    const double* Df = W.diff + (size_t)a.fbuf * N;
+    const double qK0 = last[0].x;
+    const double s1 = fl1(M, qK0) / M.T0, s2 = -rint_at(H, Fl, Df, N - 1, a.lr + 1, live), s3 = qK0 / M.T0,
+                 s4 = 2 * M.Gamma;
+    double ds1[K], ds2[K], ds3[K], ds4[K];
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+        const History Ht = tan_history(H, X, h);
+        const double tq0 = X.V[h][(size_t)a.lr * N].x;
+        ds1[h] = (fl2(M, qK0) * tq0 + X.dL[h] * fl1l(M, qK0)) / M.T0;
+        ds2[h] = -rint_at(Ht, Fl, Df, N - 1, a.lr + 1, live);
+        ds3[h] = tq0 / M.T0;
+        ds4[h] = 2 * X.dG[h];
+    }
+    double hk[EPT], hr[EPT], thk[EPT][K], thr[EPT][K];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int j = threadIdx.x + e * kStageThreads;
+        hk[e] = hr[e] = 0.0;
+#pragma unroll
+        for (int h = 0; h < K; ++h) thk[e][h] = thr[e][h] = 0.0;
+        if (j >= N) continue;
+        double C[kSums];
+#pragma unroll
+        for (int q = 0; q < kSums; ++q) C[q] = 0.0;
+        for (int c = 0; c < a.split; ++c)
+#pragma unroll
+            for (int q = 0; q < kSums; ++q) C[q] += W.partial[((size_t)j * a.split + c) * kSums + q];
+        const double th = T.theta[j];
+        const double fA = t * th, fR = t * (1 - th);
+        const double ri = rint_at(H, Fl, Df, j, a.lr + 1, live);
+        const double qK = last[j].x, qR = last[j].y, b1 = W.qkb1c0[j];
+        double d1 = b1 * s1 + qK * s2;
+        d1 = d1 + C[0] * fR;
+        d1 = d1 + C[1] * fA;
+        d1 = d1 + C[2] * fA;
+        double d2 = fl1(M, b1) * s3 + qR * s4;
+        d2 = d2 + C[3] * fR;
+        d2 = d2 + C[4] * fA;
+        d2 = d2 + C[5] * fA;
+        d2 = d2 - qK * ri;
+        hk[e] = d1 + d2 * th;
+        hr[e] = (qR * s2 + C[6] * fR) + (qR * ri - C[7] * fR) * th;
+#pragma unroll
+        for (int h = 0; h < K; ++h) {
+            double TC[kSums];
+#pragma unroll
+            for (int q = 0; q < kSums; ++q) TC[q] = 0.0;
+            for (int c = 0; c < a.split; ++c)
+#pragma unroll
+                for (int q = 0; q < kSums; ++q) TC[q] += X.partial[(((size_t)h * N + j) * a.split + c) * kSums + q];
+            const History Ht = tan_history(H, X, h);
+            const double tri = rint_at(Ht, Fl, Df, j, a.lr + 1, live);
+            const double2 tl = X.V[h][(size_t)a.lr * N + j];
+            const double tb1 = X.qkb1c0[(size_t)h * N + j];
+            const double td1 = tb1 * s1 + b1 * ds1[h] + tl.x * s2 + qK * ds2[h] + TC[0] * fR + TC[1] * fA + TC[2] * fA;
+            const double td2 = (fl2(M, b1) * tb1 + X.dL[h] * fl1l(M, b1)) * s3 + fl1(M, b1) * ds3[h] + tl.y * s4 +
+                               qR * ds4[h] + TC[3] * fR + TC[4] * fA + TC[5] * fA - tl.x * ri - qK * tri;
+            thk[e][h] = td1 + td2 * th;
+            thr[e][h] = (tl.y * s2 + qR * ds2[h] + TC[6] * fR) + (tl.y * ri + qR * tri - TC[7] * fR) * th;
+            X.hK[((size_t)h * kMaxStages + n) * N + j] = thk[e][h];
+            X.hR[((size_t)h * kMaxStages + n) * N + j] = thr[e][h];
+        }
+    }
+    // tangent drstep2 at stage 0 (time_steps.cpp:117-125)
+    double tdr[K];
+    if (n == 0) {
+        double v[6 * K];
+#pragma unroll
+        for (int q = 0; q < 6 * K; ++q) v[q] = 0.0;
+        double c0t[K];
+#pragma unroll
+        for (int h = 0; h < K; ++h) c0t[h] = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int j = threadIdx.x + e * kStageThreads;
+            if (j >= N) continue;
+            const double qK = last[j].x, qR = last[j].y, w = T.integ[j];
+            const double f1 = fl1(M, qK), f2 = fl2(M, qK), f3 = fl3(M, qK);
+            const double sK = f1, sR = f2 * qR;
+            const double dsK = f2 * hk[e], dsR = (f3 * qR) * hk[e] + f2 * hr[e];
+#pragma unroll
+            for (int h = 0; h < K; ++h) {
+                const double l = X.dL[h];
+                const double2 tl = X.V[h][(size_t)a.lr * N + j];
+                const double tf2 = f3 * tl.x + l * fl2l(M, qK);
+                const double tf3 = fl4(M, qK) * tl.x + l * fl3l(M, qK);
+                const double tsK = f2 * tl.x + l * fl1l(M, qK);
+                const double tsR = tf2 * qR + f2 * tl.y;
+                const double tdsK = tf2 * hk[e] + f2 * thk[e][h];
+                const double tdsR = (tf3 * qR + f3 * tl.y) * hk[e] + (f3 * qR) * thk[e][h] + tf2 * hr[e] + f2 * thr[e][h];
+                double* vv = v + 6 * h;
+                vv[0] += w * (tsR * qK + sR * tl.x);
+                vv[1] += w * (tsK * qR + sK * tl.y);
+                vv[2] += t * w * (tdsR * qK + dsR * tl.x);
+                vv[3] += t * w * (tdsK * qR + dsK * tl.y);
+                vv[4] += t * w * (tsR * hk[e] + sR * thk[e][h]);
+                vv[5] += t * w * (tsK * hr[e] + sK * thr[e][h]);
+                if (j == 0) c0t[h] = (tdsK * qK + dsK * tl.x + tsK * hk[e] + sK * thk[e][h]) / M.T0;
+            }
+        }
+        block_sum<6 * K, kStageThreads>(v, red);
+        // c0t is valid in thread 0 (entry j = 0), which writes the result
+#pragma unroll
+        for (int h = 0; h < K; ++h) {
+            tdr[h] = v[6 * h] + v[6 * h + 1] + v[6 * h + 2] + v[6 * h + 3] + v[6 * h + 4] + v[6 * h + 5] + c0t[h];
+            if (threadIdx.x == 0) X.scal[h * 8 + 0] = tdr[h];
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < K; ++h) tdr[h] = X.scal[h * 8 + 0];
+    }
+    // tangent stage combination, row write and rstep (time_steps.cpp:57-92,140-157)
+    const double2* g0 = H.V + (size_t)a.g0row * N;
+    double rr[2 * K];
+#pragma unroll
+    for (int q = 0; q < 2 * K; ++q) rr[q] = 0.0;
+    double c0r[K];
+#pragma unroll
+    for (int h = 0; h < K; ++h) c0r[h] = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int j = threadIdx.x + e * kStageThreads;
+        if (j >= N) continue;
+        double gK = g0[j].x, gR = g0[j].y;
+        for (int q = 0; q < a.nterms; ++q) {
+            gK = gK + ((q == n) ? hk[e] : W.hK[(size_t)q * N + j]) * a.coef[q];
+            gR = gR + ((q == n) ? hr[e] : W.hR[(size_t)q * N + j]) * a.coef[q];
+        }
+        const double w = T.integ[j];
+        const double f1 = fl1(M, gK), f2 = fl2(M, gK), f3 = fl3(M, gK);
+        const double sK = f1, sR = f2 * gR;
+#pragma unroll
+        for (int h = 0; h < K; ++h) {
+            const double2 tg0 = X.V[h][(size_t)a.g0row * N + j];
+            double tgK = tg0.x, tgR = tg0.y;
+            for (int q = 0; q < a.nterms; ++q) {
+                tgK = tgK + ((q == n) ? thk[e][h] : X.hK[((size_t)h * kMaxStages + q) * N + j]) * a.coef[q];
+                tgR = tgR + ((q == n) ? thr[e][h] : X.hR[((size_t)h * kMaxStages + q) * N + j]) * a.coef[q];
+            }
+            const double th0k = (n == 0) ? thk[e][h] : X.hK[((size_t)h * kMaxStages) * N + j];
+            const double th0r = (n == 0) ? thr[e][h] : X.hR[((size_t)h * kMaxStages) * N + j];
+            X.V[h][(size_t)a.wrow * N + j] = make_double2(tgK, tgR);
+            X.D[h][(size_t)a.wrow * N + j] = make_double2(a.tdiff * th0k, a.tdiff * th0r);
+            const double l = X.dL[h];
+            const double tsK = f2 * tgK + l * fl1l(M, gK);
+            const double tsR = (f3 * tgK + l * fl2l(M, gK)) * gR + f2 * tgR;
+            rr[2 * h] += a.t_new * w * (tsR * gK + sR * tgK);
+            rr[2 * h + 1] += a.t_new * w * (tsK * gR + sK * tgR);
+            if (j == 0) c0r[h] = (tsK * gK + sK * tgK) / M.T0;
+        }
+    }
+    block_sum<2 * K, kStageThreads>(rr, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int h = 0; h < K; ++h) {
+            X.r[h][a.wrow] = X.dG[h] + rr[2 * h] + rr[2 * h + 1] + c0r[h];
+            X.dr[h][a.wrow] = a.tdiff * tdr[h];
+        }
+    }
+}
+
+// Tangent of the energy at time t on history row `row` (simulation_runner.cpp:308-318)
+template <int K>
+__global__ __launch_bounds__(kStageThreads) void tan_energy_kernel(Tables T, History H, Model M, TanArgs X, int row,
+                                                                   double t) {
+    __shared__ double red[K * (kStageThreads / kWave)];
+    const int N = T.N;
+    const double2* v = H.V + (size_t)row * N;
+    double s[K];
+#pragma unroll
+    for (int h = 0; h < K; ++h) s[h] = 0.0;
+    for (int j = threadIdx.x; j < N; j += kStageThreads) {
+        const double qK = v[j].x, qR = v[j].y, w = T.integ[j];
+#pragma unroll
+        for (int h = 0; h < K; ++h) {
+            const double2 tv = X.V[h][(size_t)row * N + j];
+            s[h] += t * w * ((fl2(M, qK) * tv.x + X.dL[h] * fl1l(M, qK)) * qR + fl1(M, qK) * tv.y);
+        }
+    }
+    block_sum<K, kStageThreads>(s, red);
+    if (threadIdx.x == 0) {
+        const double qK0 = v[0].x;
+#pragma unroll
+        for (int h = 0; h < K; ++h) {
+            const double tq0 = X.V[h][(size_t)row * N].x;
+            X.scal[h * 8 + 2] = -(s[h] + (fl2(M, qK0) * tq0 + X.dL[h] * fl1l(M, qK0)) / M.T0);
+        }
+    }
+}
+
 // Plain streaming copy, used to calibrate HBM byte counters (FETCH_SIZE / WRITE_SIZE).
 __global__ __launch_bounds__(256) void hbm_copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
                                                        long long n) {
@@ -1152,6 +1599,55 @@ void launch_dense_apply(const double* W, const double* X, double* Y, long long P
                         hipStream_t s) {
     const long long nb = (P + kDenseRows - 1) / kDenseRows;
     hipLaunchKernelGGL(dense_apply_kernel, dim3((unsigned)nb), dim3(256), 0, s, W, X, Y, P, K, B, ldy);
+}
+
+void launch_tan_init_n(const Model& M, const TanArgs& X, int N, hipStream_t s) {
+    hipLaunchKernelGGL(tan_init_kernel, dim3(X.K), dim3(256), 0, s, M, X, N);
+}
+
+void launch_tan_rhs(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
+                    const TanArgs& X, hipStream_t s) {
+    const dim3 grid((unsigned)(T.N * a.split));
+    const int cw = (T.N + a.split - 1) / a.split;
+    const size_t lds = (size_t)(T.N + cw) * sizeof(double2) + (size_t)cw * (sizeof(double) + 2 * sizeof(int)) +
+                       (size_t)X.K * (T.N + cw) * sizeof(double2);
+#define DMFE_TR(GG, KK) \
+    hipLaunchKernelGGL((rhs_rows_tan_kernel<GG, KK>), grid, dim3(kRowThreads), lds, s, T, H, W, M, a, X)
+    if (X.K == 1) {
+        if (T.m <= 16) DMFE_TR(8, 1);
+        else if (T.m <= 32) DMFE_TR(16, 1);
+        else DMFE_TR(64, 1);
+    } else {
+        if (T.m <= 16) DMFE_TR(8, 2);
+        else if (T.m <= 32) DMFE_TR(16, 2);
+        else DMFE_TR(64, 2);
+    }
+#undef DMFE_TR
+}
+
+void launch_tan_finish(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& f,
+                       const TanArgs& X, hipStream_t s) {
+    const int ept = (T.N + kStageThreads - 1) / kStageThreads;
+#define DMFE_TF(E, KK) \
+    hipLaunchKernelGGL((tan_finish_kernel<E, KK>), dim3(1), dim3(kStageThreads), 0, s, T, H, W, M, f, X)
+    if (X.K == 1) {
+        if (ept <= 1) DMFE_TF(1, 1);
+        else if (ept <= 2) DMFE_TF(2, 1);
+        else if (ept <= 4) DMFE_TF(4, 1);
+        else DMFE_TF(8, 1);
+    } else {
+        if (ept <= 1) DMFE_TF(1, 2);
+        else if (ept <= 2) DMFE_TF(2, 2);
+        else if (ept <= 4) DMFE_TF(4, 2);
+        else DMFE_TF(8, 2);
+    }
+#undef DMFE_TF
+}
+
+void launch_tan_energy(const Tables& T, const History& H, const Model& M, const TanArgs& X, int row, double t,
+                       hipStream_t s) {
+    if (X.K == 1) hipLaunchKernelGGL(tan_energy_kernel<1>, dim3(1), dim3(kStageThreads), 0, s, T, H, M, X, row, t);
+    else hipLaunchKernelGGL(tan_energy_kernel<2>, dim3(1), dim3(kStageThreads), 0, s, T, H, M, X, row, t);
 }
 
 void launch_hbm_copy(const double2* src, double2* dst, long long n, hipStream_t s) {

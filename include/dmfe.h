@@ -209,6 +209,27 @@ int dmfe_sparsify(dmfe_ctx* ctx, double threshold);
 int dmfe_profile(dmfe_ctx* ctx, int enable);
 int dmfe_profile_read(dmfe_ctx* ctx, int kind, double* total_ms, int64_t* launches);
 
+/* ------------------------------------------------- parameter sensitivities
+ * Derivatives of the step path with respect to Gamma (the bath temperature T, -G) and
+ * lambda, carried in forward (tangent-linear) mode next to the primal run: with two
+ * parameters this gives the same gradient as an adjoint sweep without storing or
+ * recomputing checkpoints.  The time grid (dt sequence, positions) is frozen, i.e. the
+ * derivative of the discrete map of the run that is being made.  Not available in
+ * log-response mode or across a sparsification point.  The reference has no
+ * counterpart (BASELINE's adjoint requirement). */
+
+/* Starts K <= 2 tangent directions right after dmfe_init_quench: direction k perturbs
+ * (Gamma, lambda) by (dGamma[k], dLambda[k]).  Subsequent dmfe_rk_step /
+ * dmfe_loop_iteration / dmfe_run calls advance the tangents. */
+int dmfe_tangent_init(dmfe_ctx* ctx, int K, const double* dGamma, const double* dLambda);
+/* Directional derivatives of E(t) (energy at time t on the last row) and QK(t,0). */
+int dmfe_tangent_observables(dmfe_ctx* ctx, double t, double* dE, double* dQK0);
+/* Tangent history of direction k: dQK, dQR [T1*N], dr [T1] (rvec tangent); any may be NULL. */
+int dmfe_tangent_state(dmfe_ctx* ctx, int k, double* dQK, double* dQR, double* dr);
+/* Quench + run (as dmfe_run) carrying both directions; grad = [dE/dGamma, dE/dlambda,
+ * dQK0/dGamma, dQK0/dlambda] at the final proposed time *t_final. */
+int dmfe_gradients(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* grad, double* t_final, int64_t* steps);
+
 /* Dense (B-spline, m = N) stencil table `map` (0 A1, 1 A2, 2 B2) applied to B <= 16
  * right-hand sides as one FP64 MFMA contraction: Y [N*N][B] = W [N*N][N] . X [N][B]
  * (row-major host buffers).  This is indexVecLN3 with m = N (index_vec.cpp:15-61)

@@ -1,0 +1,152 @@
+"""Parameter sensitivities d(E, QK(t,0))/d(Gamma, lambda) of the step path (forward mode).
+
+The derivative is that of the discrete run with its time grid frozen, so it is checked
+against central finite differences (h = 1e-6) of runs that replay the nominal run's dt
+and integrator sequence step by step: on the GPU path itself, and on the CPU oracle (the
+reference algorithm) as the reference gradient.  A thin torch.autograd.Function wraps
+the run for the check through torch's autograd.
+"""
+import numpy as np
+import pytest
+import torch
+
+import dynamite_amd as dm
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+N, STEPS, GAMMA, LAM, H = 64, 160, 0.7, 0.5, 1e-6
+_GRID = {}
+
+
+def grid():
+    if "g" not in _GRID:
+        _GRID["g"] = dm.generate_grid(N, 1e2, 5, dm.INTERP_POLY, 9, 0)
+    return _GRID["g"]
+
+
+def params(G=GAMMA, lam=LAM):
+    return dm.default_params(p=3, p2=4, lambda_=lam, T0=1e50, Gamma=G)
+
+
+def nominal_with_tangents(dG, dL):
+    """Adaptive run from the quench carrying tangents; returns the control sequence."""
+    ctx = dm.Context(grid(), params())
+    ctx.init_quench()
+    ctx.tangent_init(dG, dL)
+    seq = []
+    for _ in range(STEPS):
+        c = ctx.get_control()
+        seq.append((c["delta_t"], c["method"]))
+        t = ctx.loop_iteration()
+    E, q0 = ctx.observables(t)
+    dE, dQ = ctx.tangent_observables(t)
+    st = ctx.get_state()
+    ctx.close()
+    return seq, t, (E, q0), (dE, dQ), st
+
+
+def replay_gpu(seq, t, G, lam):
+    ctx = dm.Context(grid(), params(G, lam))
+    ctx.init_quench()
+    for dt, method in seq:
+        ctx.set_control(dt, 1.0, 0.0, 10, method)
+        ctx.rk_step()
+    E, q0 = ctx.observables(t)
+    ctx.close()
+    return np.array([E, q0])
+
+
+def replay_cpu(seq, t, G, lam):
+    orc = Oracle(grid(), p=3, p2=4, lam=lam, T0=1e50, Gamma=G)
+    orc.init_quench()
+    for dt, method in seq:
+        orc.set_control(dt, 1.0, 0.0, 10, method)
+        orc.update()
+    E = orc.energy(t)
+    s = orc.state()
+    q0 = s["QK"][(len(s["t1grid"]) - 1) * N]
+    return np.array([E, q0])
+
+
+def fd(replay, seq, t):
+    """columns: d/dGamma, d/dlambda of (E, QK0)"""
+    J = np.zeros((2, 2))
+    for col, (dg, dl) in enumerate(((H, 0.0), (0.0, H))):
+        J[:, col] = (replay(seq, t, GAMMA + dg, LAM + dl) - replay(seq, t, GAMMA - dg, LAM - dl)) / (2 * H)
+    return J
+
+
+@pytest.fixture(scope="module")
+def nominal():
+    print(f"sensitivities: N={N}, {STEPS} steps, Gamma={GAMMA}, lambda={LAM}", flush=True)
+    return nominal_with_tangents([1.0, 0.0], [0.0, 1.0])
+
+
+def jac(nom):
+    (dE, dQ) = nom[3]
+    return np.array([[dE[0], dE[1]], [dQ[0], dQ[1]]])
+
+
+def test_tangent_run_leaves_primal_unchanged(nominal):
+    seq, t, (E, q0), _, st = nominal
+    ctx = dm.Context(grid(), params())
+    ctx.init_quench()
+    for _ in range(STEPS):
+        ctx.loop_iteration()
+    s2 = ctx.get_state()
+    ctx.close()
+    for k in ("t1grid", "QK", "QR", "rvec"):
+        assert np.array_equal(st[k], s2[k]), k
+
+
+def test_matches_gpu_finite_differences(nominal):
+    seq, t = nominal[0], nominal[1]
+    J, Jfd = jac(nominal), fd(replay_gpu, seq, t)
+    print("tangent", J.ravel(), "fd", Jfd.ravel(), flush=True)
+    assert np.all(np.abs(J - Jfd) <= 1e-6 * np.abs(Jfd).max() + 1e-9), (J, Jfd)
+
+
+def test_matches_cpu_reference_gradient(nominal):
+    seq, t = nominal[0], nominal[1]
+    J, Jcpu = jac(nominal), fd(replay_cpu, seq, t)
+    assert np.all(np.abs(J - Jcpu) <= 1e-6 * np.abs(Jcpu).max() + 1e-9), (J, Jcpu)
+
+
+def test_tangent_is_linear_in_the_seed(nominal):
+    J = jac(nominal)
+    _, _, _, (dE, dQ), _ = nominal_with_tangents([2.0], [-3.0])
+    assert abs(dE[0] - (2 * J[0, 0] - 3 * J[0, 1])) <= 1e-10 * np.abs(J[0]).sum()
+    assert abs(dQ[0] - (2 * J[1, 0] - 3 * J[1, 1])) <= 1e-10 * np.abs(J[1]).sum()
+
+
+class QuenchObservables(torch.autograd.Function):
+    """(Gamma, lambda) -> (E, QK(t,0)) of the quench run; backward from the tangents."""
+
+    @staticmethod
+    def forward(ctx_, G, lam):
+        c = dm.Context(grid(), params(float(G), float(lam)))
+        grad, t, n = c.gradients(1e300, STEPS)
+        E, q0 = c.observables(t)
+        c.close()
+        ctx_.save_for_backward(torch.tensor(grad, dtype=torch.float64))
+        return torch.tensor([E, q0], dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx_, g):
+        (grad,) = ctx_.saved_tensors
+        J = grad.reshape(2, 2)  # rows E, QK0; columns Gamma, lambda
+        v = g @ J
+        return v[0], v[1]
+
+
+def test_torch_autograd_binding(nominal):
+    seq, t = nominal[0], nominal[1]
+    G = torch.tensor(GAMMA, dtype=torch.float64, requires_grad=True)
+    lam = torch.tensor(LAM, dtype=torch.float64, requires_grad=True)
+    out = QuenchObservables.apply(G, lam)
+    (out[0] + 3.0 * out[1]).backward()
+    Jfd = fd(replay_gpu, seq, t)
+    want = Jfd[0] + 3.0 * Jfd[1]
+    got = np.array([G.grad.item(), lam.grad.item()])
+    assert np.all(np.abs(got - want) <= 1e-6 * np.abs(want).max() + 1e-9), (got, want)
