@@ -180,6 +180,67 @@ def dense_microbench(N=512, order=9, B=4, reps=10):
                 fp64_mfma_peak_TFLOPs=78.6, dense_rhs_us=rhs_ms * 1e3)
 
 
+def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6):
+    """Gradients of E and QK(t,0) at t_final w.r.t. (Gamma, lambda) for p=3+4, lambda=0.5,
+    Gamma=0.7 on an L=256 poly-9 grid (forward-mode tangents carried with the run): the
+    cost against the same run without tangents, and, on a shorter horizon, central finite
+    differences (h = 1e-6) of runs replaying the nominal dt / integrator sequence."""
+    import numpy as np
+    import dynamite_amd as dm
+    g = make_grid(256)
+
+    def prm(G=0.7, lam=0.5):
+        return dm.default_params(p=3, p2=4, lambda_=lam, T0=1e50, Gamma=G)
+    c = dm.Context(g, prm(), rows_hint=1024)
+    c.init_quench()
+    t0 = time.perf_counter()
+    n_fwd, _ = c.run(tmax, 10**12)
+    c.synchronize()
+    el_fwd = time.perf_counter() - t0
+    c.close()
+    c = dm.Context(g, prm(), rows_hint=1024)
+    t0 = time.perf_counter()
+    grad, t_final, n_grad = c.gradients(tmax, 10**12)
+    c.synchronize()
+    el_grad = time.perf_counter() - t0
+    c.close()
+    log(f"[bench] sensitivities: {n_grad} steps to t={t_final:.4g}: forward {el_fwd:.2f}s, with tangents "
+        f"{el_grad:.2f}s ({el_grad / el_fwd:.2f}x); grad {grad}")
+    # finite-difference check on a shorter horizon with the dt sequence frozen
+    c = dm.Context(g, prm(), rows_hint=1024)
+    c.init_quench()
+    c.tangent_init([1.0, 0.0], [0.0, 1.0])
+    seq, t = [], 0.0
+    while t < fd_tmax:
+        ctl = c.get_control()
+        seq.append((ctl["delta_t"], ctl["method"]))
+        t = c.loop_iteration()
+    dE, dQ = c.tangent_observables(t)
+    c.close()
+    J = np.array([[dE[0], dE[1]], [dQ[0], dQ[1]]])
+
+    def replay(G, lam):
+        r = dm.Context(g, prm(G, lam), rows_hint=len(seq) + 64)
+        r.init_quench()
+        for dt, method in seq:
+            r.set_control(dt, 1.0, 0.0, 10, method)
+            r.rk_step()
+        E, q0 = r.observables(t)
+        r.close()
+        return np.array([E, q0])
+    Jfd = np.zeros((2, 2))
+    for col, (dg, dl) in enumerate(((h, 0.0), (0.0, h))):
+        Jfd[:, col] = (replay(0.7 + dg, 0.5 + dl) - replay(0.7 - dg, 0.5 - dl)) / (2 * h)
+    err = float(np.max(np.abs(J - Jfd)) / np.max(np.abs(Jfd)))
+    log(f"[bench] sensitivities vs finite differences at t={t:.4g} ({len(seq)} steps): max rel err {err:.2e}")
+    return dict(workload=f"L=256 poly-9 grid (spline 5, Tmax 1e5), p=3+4, lambda=0.5, Gamma=0.7, quench to "
+                         f"t_final={t_final:.4g} ({n_grad} adaptive RK steps)",
+                grad={"dE_dGamma": grad[0], "dE_dlambda": grad[1], "dQK0_dGamma": grad[2], "dQK0_dlambda": grad[3]},
+                mode="forward (tangent-linear), 2 directions, dt sequence frozen",
+                forward_s=el_fwd, with_tangents_s=el_grad, cost_ratio=el_grad / el_fwd,
+                fd_check={"t_final": t, "steps": len(seq), "h": h, "max_rel_err": err})
+
+
 BATCH_SEED = 20261203
 
 
@@ -281,6 +342,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--no-sensitivity", action="store_true")
     ap.add_argument("--batch-prep", type=int, default=1000)
     ap.add_argument("--batch-steps", type=int, default=64)
     ap.add_argument("--looped-steps", type=int, default=10)
@@ -349,6 +411,8 @@ def main():
         log(f"[bench] dense: {out['dense_bspline']}")
     if not a.no_batch:
         out["batch"] = batch_run(a.batch_prep, a.batch_steps, a.looped_steps)
+    if not a.no_sensitivity:
+        out["sensitivities"] = sensitivity_run()
     if not a.no_cpu:
         log(f"[bench] CPU baseline (budget {a.cpu_budget:.0f}s)")
         out["cpu_baseline"] = cpu_baseline(g, state, control, a.cpu_budget)
