@@ -280,6 +280,16 @@ class Context:
     def sparsify(self, threshold):
         _check(lib().dmfe_sparsify(self._h, C.c_double(threshold)), "dmfe_sparsify")
 
+    def save_state(self, path):
+        """data.bin version-1 checkpoint (the reference's binary layout)."""
+        _check(lib().dmfe_save_state(self._h, os.fsencode(path)), "dmfe_save_state")
+
+    def load_state(self, path):
+        """Resume from a data.bin version-1 checkpoint; returns the energy stored in it."""
+        e = C.c_double()
+        _check(lib().dmfe_load_state(self._h, os.fsencode(path), C.byref(e)), "dmfe_load_state")
+        return e.value
+
     # -- parameter sensitivities (forward mode) ------------------------------------
     def tangent_init(self, dGamma, dLambda):
         g = np.ascontiguousarray(dGamma, np.float64).ravel()

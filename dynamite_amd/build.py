@@ -3,7 +3,8 @@
     python -m dynamite_amd.build          # or: python dynamite_amd/build.py
 
 Compiles in-tree with hipcc for gfx950 only.  Objects go to dynamite_amd/_build/,
-the shared library to dynamite_amd/libdmfe.so.
+the shared library to dynamite_amd/libdmfe.so, the command-line driver (g++ over the C
+ABI) to dynamite_amd/dmfe_run.
 """
 import os
 import shutil
@@ -14,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdmfe.so")
+DRIVER = os.path.join(HERE, "dmfe_run")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -52,6 +54,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     shutil.move(tmp, LIB)
+    # the command-line driver: a plain C++ consumer of the C ABI, linked against libdmfe.so
+    drv_tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(HERE, '..', 'include')}", "-o", drv_tmp,
+          os.path.join(CSRC, "dmfe_driver.cpp"), f"-L{HERE}", "-ldmfe", "-Wl,-rpath,$ORIGIN"])
+    shutil.move(drv_tmp, DRIVER)
     return LIB
 
 

@@ -999,6 +999,92 @@ extern "C" int dmfe_gradients(dmfe_ctx* c, double tmax, int64_t max_loop, double
     return DMFE_OK;
 }
 
+// data.bin version 1 (io_save_binary.cpp:66-100): int version, size_t T1, size_t N,
+// int p, int p2, double lambda, T0, Gamma, delta, delta_t, int loop, double energy, then
+// t1grid [T1], QK, QR, dQK, dQR [T1*N], rvec, drvec [T1]; raw, unpadded, native endian.
+extern "C" int dmfe_save_state(dmfe_ctx* c, const char* path) {
+    if (!c || !path || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_save_state: no state or null path");
+    const size_t N = (size_t)c->N, n = (size_t)c->T1;
+    // energy at t1grid.back() on the last row, as the writer computes it
+    double energy = 0.0, qk0 = 0.0;
+    int rc = observables(c, c->t1[n - 1], &energy, &qk0);
+    if (rc) return rc;
+    std::vector<double> QK(n * N), QR(n * N), dQK(n * N), dQR(n * N), rv(n), drv(n);
+    rc = dmfe_get_state(c, nullptr, QK.data(), QR.data(), dQK.data(), dQR.data(), rv.data(), drv.data(), nullptr);
+    if (rc) return rc;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_error(DMFE_ERR_IO, std::string("dmfe_save_state: cannot open ") + path);
+    const int version = 1;
+    const size_t t1size = n, len = N;
+    const int p = c->params.p, p2 = c->params.p2, loop = (int)c->loop;
+    bool ok = std::fwrite(&version, sizeof(int), 1, f) == 1 && std::fwrite(&t1size, sizeof(size_t), 1, f) == 1 &&
+              std::fwrite(&len, sizeof(size_t), 1, f) == 1 && std::fwrite(&p, sizeof(int), 1, f) == 1 &&
+              std::fwrite(&p2, sizeof(int), 1, f) == 1 &&
+              std::fwrite(&c->params.lambda, sizeof(double), 1, f) == 1 &&
+              std::fwrite(&c->params.T0, sizeof(double), 1, f) == 1 &&
+              std::fwrite(&c->params.Gamma, sizeof(double), 1, f) == 1 &&
+              std::fwrite(&c->delta, sizeof(double), 1, f) == 1 && std::fwrite(&c->dt, sizeof(double), 1, f) == 1 &&
+              std::fwrite(&loop, sizeof(int), 1, f) == 1 && std::fwrite(&energy, sizeof(double), 1, f) == 1 &&
+              std::fwrite(c->t1.data(), sizeof(double), n, f) == n;
+    for (const std::vector<double>* a : {&QK, &QR, &dQK, &dQR, &rv, &drv})
+        ok = ok && std::fwrite(a->data(), sizeof(double), a->size(), f) == a->size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return set_error(DMFE_ERR_IO, std::string("dmfe_save_state: write failed: ") + path);
+    return DMFE_OK;
+}
+
+// loadSimulationStateBinary (io_input.cpp:112-175) and the resume branch of init()
+// (initialization.cpp:225-228,277-283): parameters must match (p, p2 exactly; lambda, T0,
+// Gamma within 1e-10); delta, delta_t and loop come from the file, delta_old = 0, the
+// integrator follows from delta_t, and the next step re-interpolates.
+extern "C" int dmfe_load_state(dmfe_ctx* c, const char* path, double* energy_out) {
+    if (!c || !path) return set_error(DMFE_ERR_ARG, "dmfe_load_state: null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_error(DMFE_ERR_IO, std::string("dmfe_load_state: cannot open ") + path);
+    int version = 0, p = 0, p2 = 0, loop = 0;
+    size_t t1size = 0, len = 0;
+    double lambda = 0, T0 = 0, Gamma = 0, delta = 0, dt = 0, energy = 0;
+    bool ok = std::fread(&version, sizeof(int), 1, f) == 1 && std::fread(&t1size, sizeof(size_t), 1, f) == 1 &&
+              std::fread(&len, sizeof(size_t), 1, f) == 1 && std::fread(&p, sizeof(int), 1, f) == 1 &&
+              std::fread(&p2, sizeof(int), 1, f) == 1 && std::fread(&lambda, sizeof(double), 1, f) == 1 &&
+              std::fread(&T0, sizeof(double), 1, f) == 1 && std::fread(&Gamma, sizeof(double), 1, f) == 1 &&
+              std::fread(&delta, sizeof(double), 1, f) == 1 && std::fread(&dt, sizeof(double), 1, f) == 1 &&
+              std::fread(&loop, sizeof(int), 1, f) == 1 && std::fread(&energy, sizeof(double), 1, f) == 1;
+    if (!ok || version != 1) {
+        std::fclose(f);
+        return set_error(DMFE_ERR_IO, std::string("dmfe_load_state: not a version-1 data.bin: ") + path);
+    }
+    if (len != (size_t)c->N || t1size < 1 || t1size > ((size_t)1 << 31)) {
+        std::fclose(f);
+        return set_error(DMFE_ERR_STATE, "dmfe_load_state: file grid length " + std::to_string(len) + " != context N " +
+                                             std::to_string(c->N));
+    }
+    if (p != c->params.p || p2 != c->params.p2 || std::fabs(lambda - c->params.lambda) > 1e-10 ||
+        std::fabs(T0 - c->params.T0) > 1e-10 || std::fabs(Gamma - c->params.Gamma) > 1e-10) {
+        std::fclose(f);
+        return set_error(DMFE_ERR_STATE, "dmfe_load_state: file parameters don't match the context parameters");
+    }
+    const size_t n = t1size, N = len;
+    std::vector<double> t1(n), QK(n * N), QR(n * N), dQK(n * N), dQR(n * N), rv(n), drv(n);
+    for (std::vector<double>* a : {&t1, &QK, &QR, &dQK, &dQR, &rv, &drv})
+        ok = ok && std::fread(a->data(), sizeof(double), a->size(), f) == a->size();
+    std::fclose(f);
+    if (!ok) return set_error(DMFE_ERR_IO, std::string("dmfe_load_state: truncated file: ") + path);
+    int rc = dmfe_set_state(c, (int64_t)n, t1.data(), QK.data(), QR.data(), dQK.data(), dQR.data(), rv.data(), drv.data(),
+                            nullptr);
+    if (rc) return rc;
+    c->delta = delta;
+    c->dt = dt;
+    c->delta_old = 0.0;
+    c->loop = loop;
+    c->method = (c->dt < c->params.rmax[0] / c->specRad) ? DMFE_RK54 : DMFE_SSPRK104;
+    rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (energy_out) *energy_out = energy;
+    return DMFE_OK;
+}
+
 extern "C" int dmfe_time_rhs(dmfe_ctx* c, int reps, double* ms_out) {
     if (!c || reps < 1 || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_time_rhs: bad argument");
     if (!c->interp_valid) return set_error(DMFE_ERR_STATE, "dmfe_time_rhs: no interpolation state");

@@ -152,6 +152,15 @@ int dmfe_set_state(dmfe_ctx* ctx, int64_t T1, const double* t1grid, const double
 int dmfe_get_state(dmfe_ctx* ctx, double* t1grid, double* QK, double* QR, double* dQK, double* dQR,
                    double* rvec, double* drvec, double* dtratio);
 
+/* Checkpoints in the reference's data.bin version-1 layout (io_save_binary.cpp:66-100):
+ * dmfe_save_state writes the header (parameters, delta, delta_t, loop, energy at
+ * t1grid.back()) and the history arrays; dmfe_load_state reads such a file into the
+ * context (loadSimulationStateBinary, io_input.cpp:112-175, with its parameter check)
+ * and sets the resumed control as init() does (initialization.cpp:225-228,277-283).
+ * *energy_out (may be NULL) = the energy stored in the file. */
+int dmfe_save_state(dmfe_ctx* ctx, const char* path);
+int dmfe_load_state(dmfe_ctx* ctx, const char* path, double* energy_out);
+
 /* Control scalars (config.delta_t, delta, delta_old, loop, rk->init). */
 int dmfe_set_control(dmfe_ctx* ctx, double delta_t, double delta, double delta_old, int64_t loop, int method);
 int dmfe_get_control(const dmfe_ctx* ctx, double* delta_t, double* delta, double* delta_old, int64_t* loop,

@@ -58,3 +58,10 @@ def test_bad_grid_rejected_before_device_use():
     g.inds[0][5] = 14  # window 14..19 exceeds N = 16
     with pytest.raises(dm.DmfeError, match="exceeds N"):
         dm.Context(g, dm.default_params())
+
+
+def test_driver_help_runs_without_a_device():
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "dynamite_amd", "dmfe_run")
+    r = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "--grid" in r.stdout
