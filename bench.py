@@ -64,6 +64,12 @@ def gpu_run(g, prep, steps, warmup):
     ctx.synchronize()
     log(f"[bench] prepared a live state: {n} steps in {time.perf_counter() - t0:.2f}s, T1={ctx.T1}, "
         f"control={ctx.get_control()}")
+    # the autotuner picks the RHS kernel shape (lanes per B2 entry, column chunks) on
+    # this live history and installs it for the L=512 class
+    best, table = dm.autotune(ctx, reps=20)
+    tune = dict(shape={"G": best[0], "split": best[1]},
+                candidates_us={f"G{G}_split{sp}": round(1e3 * ms, 2) for (G, sp), ms in table.items()})
+    log(f"[bench] autotune: best G={best[0]} split={best[1]}; {tune['candidates_us']}")
     for _ in range(warmup):
         ctx.loop_iteration()
     ctx.synchronize()
@@ -81,7 +87,7 @@ def gpu_run(g, prep, steps, warmup):
     rhs_b2b_ms = dm.time_rhs(ctx, 200)
     T1 = ctx.T1
     ctx.close()
-    return el, prof, state, control, T1, rhs_b2b_ms
+    return el, prof, state, control, T1, rhs_b2b_ms, tune
 
 
 def cpu_baseline(g, state, control, budget_s=15.0):
@@ -276,6 +282,9 @@ def batch_run(prep, steps, looped_steps, chunk=16):
     b.run(prep, use_graph=True, chunk=chunk)
     log(f"[bench] batch of {n}: {prep} prep steps in {time.perf_counter() - t0:.2f}s, "
         f"T1 {min(b.T1(i) for i in range(n))}..{max(b.T1(i) for i in range(n))}")
+    # the batch autotuner picks the lane-group width of the batch RHS on these states
+    G_b, table_b = b.autotune(reps=4)
+    log(f"[bench] batch autotune: G={G_b}; ms per RHS launch {table_b}")
     b.run(2 * chunk, use_graph=True, chunk=chunk)  # warm-up, graph instantiation
     loops0 = np.array([b.get_control(i)["loop"] for i in range(n)])
     states = [(b.get_state(i), b.get_control(i)) for i in range(n)]
@@ -288,17 +297,21 @@ def batch_run(prep, steps, looped_steps, chunk=16):
     batch_rate = inst_steps / el
     log(f"[bench] batch: {nb} batch steps, {inst_steps} instance-steps in {el:.3f}s -> {batch_rate:.0f}/s "
         f"({1e3 * el / max(nb, 1):.2f} ms per batch step)")
-    # looped plain path on the same states
-    ctxs = []
+    # looped plain path on the same states; the first context of each grid size is
+    # autotuned and installs its shape for the contexts created after it
+    ctxs, tuned = [], {}
     for i in range(n):
         st, c = states[i]
         ctx = dm.Context(gl[i], pl[i], rows_hint=len(st["t1grid"]) + looped_steps + 64)
         ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
         ctx.set_control(c["delta_t"], c["delta"], c["delta_old"], c["loop"], c["method"])
         ctx.interpolate(dm.POS_BSEARCH)
+        if gl[i].N not in tuned:
+            tuned[gl[i].N] = dm.autotune(ctx, reps=10)[0]
         ctx.loop_iteration()  # warm-up
         ctx.synchronize()
         ctxs.append(ctx)
+    log(f"[bench] looped path shapes (G, split) by L: {tuned}")
     t0 = time.perf_counter()
     for ctx in ctxs:
         for _ in range(looped_steps):
@@ -312,6 +325,8 @@ def batch_run(prep, steps, looped_steps, chunk=16):
     return dict(instances=n, value=batch_rate, unit="instance-steps/s", looped_value=looped_rate,
                 ratio=batch_rate / looped_rate, batch_steps=nb, instance_steps=inst_steps,
                 ms_per_batch_step=1e3 * el / max(nb, 1), prep_steps=prep, graph_chunk=chunk,
+                autotune={"batch_G": G_b, "batch_rhs_ms": {f"G{G}": round(ms, 4) for G, ms in table_b.items()},
+                          "looped_shapes": {f"L{L}": {"G": sh[0], "split": sh[1]} for L, sh in tuned.items()}},
                 workload=f"{n} quenches, Gamma~U[0.3,1], lambda~U[0,1], L 50/50 in {{256,512}} "
                          f"(seed {BATCH_SEED}), p=3 q=4 T0=inf, poly-9 grids (spline 5, Tmax 1e5); "
                          f"timed from the live states after {prep} batch steps")
@@ -368,7 +383,7 @@ def main():
     N = 512
     g = make_grid(N)
     log(f"[bench] GPU: {a.prep} prep + {a.warmup} warmup + {a.steps} timed steps")
-    el, prof, state, control, T1, rhs_b2b_ms = gpu_run(g, a.prep, a.steps, a.warmup)
+    el, prof, state, control, T1, rhs_b2b_ms, tune = gpu_run(g, a.prep, a.steps, a.warmup)
     ms_step = 1e3 * el / a.steps
     rhs_ms, rhs_n = prof[1]
     avg_rhs_in_steps_s = rhs_ms / 1e3 / max(rhs_n, 1)
@@ -403,6 +418,7 @@ def main():
                      "kernel": "rhs_rows_kernel", "bytes_per_launch": B,
                      "avg_launch_us": avg_rhs_s * 1e6, "avg_launch_us_in_steps": avg_rhs_in_steps_s * 1e6,
                      "launches_in_steps": rhs_n},
+        "autotune": tune,
     }
     out["kernel_microbench"] = kernel_microbench(g)
     log(f"[bench] microbench: {out['kernel_microbench']}")

@@ -44,6 +44,17 @@ class Params(C.Structure):
                 ("rmax", C.c_double * 11), ("log_response", C.c_int32), ("maint_every", C.c_int32)]
 
 
+class RhsShape(C.Structure):
+    """dmfe_rhs_shape: G lanes per B2 entry, split column chunks per row."""
+    _fields_ = [("G", C.c_int32), ("split", C.c_int32)]
+
+    def __repr__(self):
+        return f"RhsShape(G={self.G}, split={self.split})"
+
+    def astuple(self):
+        return (self.G, self.split)
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
@@ -340,6 +351,41 @@ def time_rhs(ctx, reps=100):
     return ms.value
 
 
+def rhs_shape(ctx):
+    """(G, split) of the context's RHS kernel."""
+    sh = RhsShape()
+    _check(lib().dmfe_get_rhs_shape(ctx._h, C.byref(sh)), "dmfe_get_rhs_shape")
+    return sh.astuple()
+
+
+def set_rhs_shape(ctx, G, split):
+    _check(lib().dmfe_set_rhs_shape(ctx._h, C.byref(RhsShape(G, split))), "dmfe_set_rhs_shape")
+
+
+def tuning_get(N, m, log_response=0):
+    """(G, split) that contexts and batches of grid class (N, m, log_response) are created with."""
+    sh = RhsShape()
+    _check(lib().dmfe_tuning_get(C.c_int64(N), C.c_int(m), C.c_int(log_response), C.byref(sh)), "dmfe_tuning_get")
+    return sh.astuple()
+
+
+def tuning_set(N, m, log_response, G, split):
+    _check(lib().dmfe_tuning_set(C.c_int64(N), C.c_int(m), C.c_int(log_response), C.byref(RhsShape(G, split))),
+           "dmfe_tuning_set")
+
+
+def autotune(ctx, reps=20, install=True):
+    """Times every valid RHS shape on the context's current history and keeps the fastest.
+    Returns (best (G, split), {(G, split): average launch ms})."""
+    best = RhsShape()
+    tried = (RhsShape * 16)()
+    ms = (C.c_double * 16)()
+    n = C.c_int()
+    _check(lib().dmfe_autotune(ctx._h, C.c_int(reps), C.c_int(1 if install else 0), C.byref(best), tried, ms,
+                               C.c_int(16), C.byref(n)), "dmfe_autotune")
+    return best.astuple(), {tried[q].astuple(): ms[q] for q in range(min(n.value, 16))}
+
+
 def dense_apply(ctx, X, map_=0, reps=1):
     """Y = W_map . X for a dense (B-spline) grid context on MFMA; X is (N, B) with B <= 16.
     Returns (Y of shape (N*N, B), average device ms per contraction)."""
@@ -386,6 +432,27 @@ class Batch:
         if self._h:
             lib().dmfe_batch_destroy(self._h)
             self._h = C.c_void_p()
+
+    @property
+    def lanes(self):
+        """Lane-group width G of the batch RHS kernel."""
+        G = C.c_int()
+        _check(lib().dmfe_batch_get_lanes(self._h, C.byref(G)), "dmfe_batch_get_lanes")
+        return G.value
+
+    @lanes.setter
+    def lanes(self, G):
+        _check(lib().dmfe_batch_set_lanes(self._h, C.c_int(G)), "dmfe_batch_set_lanes")
+
+    def autotune(self, reps=5):
+        """Times the batch RHS for every valid G and keeps the fastest: (G, {G: ms})."""
+        best = C.c_int()
+        tried = (C.c_int32 * 8)()
+        ms = (C.c_double * 8)()
+        n = C.c_int()
+        _check(lib().dmfe_batch_autotune(self._h, C.c_int(reps), C.byref(best), tried, ms, C.c_int(8), C.byref(n)),
+               "dmfe_batch_autotune")
+        return best.value, {tried[q]: ms[q] for q in range(min(n.value, 8))}
 
     def __del__(self):
         try:

@@ -89,7 +89,7 @@ struct dmfe_batch {
     Ctl* d_ctl = nullptr;
     int2* d_smap = nullptr;
     int4* d_rmap = nullptr;
-    int nb_stage = 0, nb_rhs = 0, ept = 1, G = 8, log_response = 0;
+    int nb_stage = 0, nb_rhs = 0, ept = 1, G = 8, maxm = 1, log_response = 0;
     size_t lds_rhs = 0;
     double* d_obs = nullptr;
     int obs_cap = 0;
@@ -165,7 +165,7 @@ int enqueue_step(dmfe_batch* b, int S) {
     launch_batch_begin(b->d_desc, n, b->stream);
     for (int k = 0; k < S; ++k) {
         launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);
-        launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->lds_rhs, b->stream);
+        launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->maxm, b->lds_rhs, b->stream);
     }
     launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, S, b->ept, b->stream);
     launch_batch_end(b->d_desc, n, b->stream);
@@ -439,18 +439,23 @@ extern "C" int dmfe_batch_create(int n, const dmfe_grid* grids, const dmfe_param
         x.M.lambda = params[i].lambda; x.M.T0 = params[i].T0; x.M.Gamma = params[i].Gamma;
         x.M.p = params[i].p; x.M.p2 = params[i].p2;
         const int N = (int)g->N;
-        x.split = (N >= 256) ? std::max(1, N / 256) : 1;
+        const RhsShape sh = rhs_shape_lookup(N, (int)g->m, params[0].log_response);
+        x.split = sh.split;
+        if (i == 0) b->G = sh.G;
         x.cap = cap;
         int rc = alloc_hist(x.H, cap, N);
         if (rc) return fail(rc);
-        rc = alloc_work(x.W, N, x.split);
+        rc = alloc_work(x.W, N, rhs_max_split(N));
         if (rc) return fail(rc);
         maxm = std::max(maxm, (int)g->m);
         const int cw = (N + x.split - 1) / x.split;
-        b->lds_rhs = std::max(b->lds_rhs, (size_t)(N + cw) * sizeof(double2) + (size_t)cw * (sizeof(double) + 2 * sizeof(int)));
+        b->lds_rhs = std::max(b->lds_rhs, rhs_lds_bytes(N, cw, 0));
         b->ept = std::max(b->ept, (N + 255) / 256);
     }
-    b->G = maxm <= 16 ? 8 : (maxm <= 32 ? 16 : 64);
+    // one lane-group width for the whole batch (its first instance's class); widened if
+    // another instance's stencil needs it
+    if (!rhs_shape_valid(b->G, maxm)) b->G = rhs_default_G(maxm);
+    b->maxm = maxm;
     // stage map: finish blocks, theta blocks, then the phi2 rows of every instance
     std::vector<int2> smap;
     for (int i = 0; i < n; ++i) smap.push_back(make_int2(i, -2));
@@ -618,6 +623,76 @@ extern "C" int dmfe_batch_run(dmfe_batch* b, int64_t max_steps, int use_graph, i
     }
     b->steps_run += done;
     if (steps_out) *steps_out = done;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_get_lanes(const dmfe_batch* b, int* G) {
+    if (!b || !G) return set_error(DMFE_ERR_ARG, "dmfe_batch_get_lanes: null argument");
+    *G = b->G;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_batch_set_lanes(dmfe_batch* b, int G) {
+    if (!b) return set_error(DMFE_ERR_ARG, "dmfe_batch_set_lanes: null batch");
+    if (!rhs_shape_valid(G, b->maxm))
+        return set_error(DMFE_ERR_ARG, "dmfe_batch_set_lanes: G must be 4 or 8 (m <= 16), 16 (m <= 32) or 64; got " +
+                                           std::to_string(G) + " for m = " + std::to_string(b->maxm));
+    if (G != b->G) {
+        b->G = G;
+        drop_graphs(b);  // captured launches carry the old kernel
+    }
+    return DMFE_OK;
+}
+
+// Times `reps` launches of the batch RHS (stage 0 of the current step arguments, all
+// instances) for every valid lane-group width G, best of two sweeps, and keeps the
+// fastest.  Only the RHS scratch (quadrature partials) is written; the next step's RHS
+// rewrites it before use, so the batch state is unchanged.
+extern "C" int dmfe_batch_autotune(dmfe_batch* b, int reps, int* G_best, int32_t* tried_G, double* tried_ms,
+                                   int max_tried, int* n_tried) {
+    if (!b || reps < 1 || max_tried < 0) return set_error(DMFE_ERR_ARG, "dmfe_batch_autotune: bad argument");
+    if (b->steps_run < 1) return set_error(DMFE_ERR_STATE, "dmfe_batch_autotune: run at least one batch step first");
+    std::vector<int> cand;
+    for (int G : {4, 8, 16, 64})
+        if (rhs_shape_valid(G, b->maxm)) cand.push_back(G);
+    std::vector<double> ms(cand.size(), std::numeric_limits<double>::infinity());
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(DMFE_ERR_HIP, "dmfe_batch_autotune: event create failed");
+    }
+    hipError_t e = hipSuccess;
+    for (int round = 0; round < 2 && e == hipSuccess; ++round)
+        for (size_t q = 0; q < cand.size() && e == hipSuccess; ++q) {
+            launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, 0, b->log_response, cand[q], b->maxm, b->lds_rhs, b->stream);
+            e = hipEventRecord(e0, b->stream);
+            for (int r = 0; r < reps && e == hipSuccess; ++r) {
+                launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, 0, b->log_response, cand[q], b->maxm, b->lds_rhs,
+                                 b->stream);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (e == hipSuccess) ms[q] = std::min(ms[q], (double)t / reps);
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("dmfe_batch_autotune: ") + hipGetErrorString(e));
+    size_t qb = 0;
+    for (size_t q = 1; q < cand.size(); ++q)
+        if (ms[q] < ms[qb]) qb = q;
+    int rc = dmfe_batch_set_lanes(b, cand[qb]);
+    if (rc) return rc;
+    if (G_best) *G_best = cand[qb];
+    const int nt = (int)std::min<size_t>(cand.size(), (size_t)max_tried);
+    for (int q = 0; q < nt; ++q) {
+        if (tried_G) tried_G[q] = cand[(size_t)q];
+        if (tried_ms) tried_ms[q] = ms[(size_t)q];
+    }
+    if (n_tried) *n_tried = (int)cand.size();
     return DMFE_OK;
 }
 

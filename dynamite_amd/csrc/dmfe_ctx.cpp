@@ -13,7 +13,10 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace dmfe {
@@ -31,6 +34,22 @@ int set_error(int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                                             \
             return set_error(DMFE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));            \
     } while (0)
+
+// process-wide RHS shapes per grid class (N, m, log_response)
+static std::mutex g_shape_mu;
+static std::map<std::tuple<int, int, int>, RhsShape> g_shapes;
+
+RhsShape rhs_shape_lookup(int N, int m, int log_response) {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_shapes.find(std::make_tuple(N, m, log_response ? 1 : 0));
+    if (it != g_shapes.end()) return it->second;
+    return RhsShape{rhs_default_G(m), rhs_default_split(N)};
+}
+
+void rhs_shape_store(int N, int m, int log_response, RhsShape s) {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shapes[std::make_tuple(N, m, log_response ? 1 : 0)] = s;
+}
 
 // Dormand-Prince 5(4) and SSPRK(10,4) tableaux (runge_kutta.cpp:22-35,149-159)
 const Tableau& dp54() {
@@ -105,6 +124,7 @@ struct dmfe_ctx {
     int N = 0, m = 0;
     long long P = 0;
     int split = 2;
+    int G = 8;                // lanes per B2 entry of the RHS kernel
     bool dense_mfma = false;  // dense (B-spline) grid: A1/A2 contracted on MFMA, weights entry-major
     // forward-mode sensitivities (dmfe_tangent_init): tangent histories and work
     int tanK = 0;
@@ -293,6 +313,7 @@ int run_rhs_rows(dmfe_ctx* c, double* fields) {
     a.log_response = c->params.log_response;
     a.fields = fields;
     a.fbuf = c->fcur;
+    a.G = c->G;
     prof_begin(c, kKindRhs);
     if (c->dense_mfma) {
         // A1/A2 of a dense grid: (P x N) . (N x 2) on MFMA against the last row (QK, QR)
@@ -601,7 +622,11 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
     c->model.lambda = prm->lambda; c->model.T0 = prm->T0; c->model.Gamma = prm->Gamma;
     c->model.p = prm->p; c->model.p2 = prm->p2;
     c->N = N; c->m = m; c->P = P;
-    c->split = (N >= 256) ? std::max(1, N / 256) : 1;
+    {
+        const RhsShape sh = rhs_shape_lookup(N, m, prm->log_response);
+        c->G = sh.G;
+        c->split = sh.split;
+    }
     c->dense_mfma = (m == N) && !prm->log_response;
     c->specRad = 4 * std::sqrt(host_fl(*prm, 1, 2));
     auto fail = [&](int code, const std::string& msg) { free_ctx(c); return set_error(code, msg); };
@@ -616,7 +641,7 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
          hipMalloc(&c->W.saveB1, 3 * bN) == hipSuccess && hipMalloc(&c->W.saveB2, 3 * bP) == hipSuccess &&
          hipMalloc(&c->W.Floor, 2 * (size_t)N * sizeof(int)) == hipSuccess && hipMalloc(&c->W.diff, 2 * bN) == hipSuccess &&
          hipMalloc(&c->W.rInt, bN) == hipSuccess &&
-         hipMalloc(&c->W.partial, bN * (size_t)c->split * kSums) == hipSuccess &&
+         hipMalloc(&c->W.partial, bN * (size_t)rhs_max_split(N) * kSums) == hipSuccess &&
          hipMalloc(&c->W.qkb1c0, bN) == hipSuccess && hipMalloc(&c->W.hK, bN * kMaxStages) == hipSuccess &&
          hipMalloc(&c->W.hR, bN * kMaxStages) == hipSuccess && hipMalloc(&c->W.scal, 8 * sizeof(double)) == hipSuccess &&
          hipHostMalloc((void**)&c->pinned, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
@@ -1106,6 +1131,96 @@ extern "C" int dmfe_time_rhs(dmfe_ctx* c, int reps, double* ms_out) {
     if (rc) return rc;
     if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("dmfe_time_rhs: ") + hipGetErrorString(e));
     if (ms_out) *ms_out = ms / reps;
+    return DMFE_OK;
+}
+
+// ---- RHS shapes and the autotuner ---------------------------------------------------------
+static int check_shape(int64_t N, int m, const dmfe_rhs_shape* s, const char* who) {
+    if (!s) return set_error(DMFE_ERR_ARG, std::string(who) + ": null shape");
+    if (!rhs_shape_valid(s->G, m))
+        return set_error(DMFE_ERR_ARG, std::string(who) + ": G must be 4 or 8 (m <= 16), 16 (m <= 32) or 64; got G=" +
+                                           std::to_string(s->G) + " for m=" + std::to_string(m));
+    if (s->split < 1 || s->split > rhs_max_split((int)N))
+        return set_error(DMFE_ERR_ARG, std::string(who) + ": split must be in [1, " +
+                                           std::to_string(rhs_max_split((int)N)) + "]");
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_get_rhs_shape(const dmfe_ctx* c, dmfe_rhs_shape* out) {
+    if (!c || !out) return set_error(DMFE_ERR_ARG, "dmfe_get_rhs_shape: null argument");
+    out->G = c->G;
+    out->split = c->split;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_set_rhs_shape(dmfe_ctx* c, const dmfe_rhs_shape* s) {
+    if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_rhs_shape: null context");
+    if (int rc = check_shape(c->N, c->m, s, "dmfe_set_rhs_shape")) return rc;
+    if (c->tanK) return set_error(DMFE_ERR_STATE, "dmfe_set_rhs_shape: set the shape before dmfe_tangent_init");
+    c->G = s->G;
+    c->split = s->split;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_tuning_get(int64_t N, int m, int log_response, dmfe_rhs_shape* out) {
+    if (N < 1 || m < 1 || !out) return set_error(DMFE_ERR_ARG, "dmfe_tuning_get: bad argument");
+    const RhsShape sh = rhs_shape_lookup((int)N, m, log_response);
+    out->G = sh.G;
+    out->split = sh.split;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_tuning_set(int64_t N, int m, int log_response, const dmfe_rhs_shape* s) {
+    if (N < 1 || m < 1) return set_error(DMFE_ERR_ARG, "dmfe_tuning_set: bad argument");
+    if (int rc = check_shape(N, m, s, "dmfe_tuning_set")) return rc;
+    rhs_shape_store((int)N, m, log_response, RhsShape{s->G, s->split});
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_autotune(dmfe_ctx* c, int reps, int install, dmfe_rhs_shape* best, dmfe_rhs_shape* tried,
+                             double* tried_ms, int max_tried, int* n_tried) {
+    if (!c || reps < 1 || max_tried < 0) return set_error(DMFE_ERR_ARG, "dmfe_autotune: bad argument");
+    if (c->tanK) return set_error(DMFE_ERR_STATE, "dmfe_autotune: tune before dmfe_tangent_init");
+    if (!c->interp_valid || c->T1 < 2 || !(c->t1[c->T1 - 1] > 0.0))
+        return set_error(DMFE_ERR_STATE, "dmfe_autotune: needs an interpolated history with t > 0");
+    std::vector<RhsShape> cand;
+    for (int G : {4, 8, 16, 64})
+        if (rhs_shape_valid(G, c->m))
+            for (int sp : {rhs_default_split(c->N), rhs_max_split(c->N)}) cand.push_back(RhsShape{G, sp});
+    const RhsShape keep{c->G, c->split};
+    std::vector<double> ms(cand.size(), std::numeric_limits<double>::infinity());
+    for (int round = 0; round < 2; ++round)  // best of two sweeps
+        for (size_t q = 0; q < cand.size(); ++q) {
+            c->G = cand[q].G;
+            c->split = cand[q].split;
+            double t = 0.0;
+            const int rc = dmfe_time_rhs(c, reps, &t);
+            if (rc) {
+                c->G = keep.G;
+                c->split = keep.split;
+                return rc;
+            }
+            ms[q] = std::min(ms[q], t);
+        }
+    size_t qb = 0;
+    for (size_t q = 1; q < cand.size(); ++q)
+        if (ms[q] < ms[qb]) qb = q;
+    c->G = cand[qb].G;
+    c->split = cand[qb].split;
+    if (install) rhs_shape_store(c->N, c->m, c->params.log_response, cand[qb]);
+    if (best) {
+        best->G = cand[qb].G;
+        best->split = cand[qb].split;
+    }
+    const int nt = (int)std::min<size_t>(cand.size(), (size_t)max_tried);
+    for (int q = 0; q < nt; ++q) {
+        if (tried) {
+            tried[q].G = cand[(size_t)q].G;
+            tried[q].split = cand[(size_t)q].split;
+        }
+        if (tried_ms) tried_ms[q] = ms[(size_t)q];
+    }
+    if (n_tried) *n_tried = (int)cand.size();
     return DMFE_OK;
 }
 

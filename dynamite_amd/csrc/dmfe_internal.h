@@ -26,6 +26,17 @@ void grid_bspline_weights(const std::vector<long double>& x, const std::vector<l
 constexpr int kMaxN = 2048;     // LDS sizing of the row kernels
 constexpr int kMaxStages = 10;  // SSPRK(10,4)
 constexpr int kSums = 8;        // quadratures per RHS row
+// RHS rows with N <= kTailMaxN keep the last two history rows (V[T1-2], D[T1-1]) in LDS
+// beside V[T1-1], so B2 targets in the last time interval read the block's LDS copy.
+constexpr int kTailMaxN = 1024;
+
+// dynamic LDS of one RHS row block: [N] V[T1-1], ([N] V[T1-2], [N] D[T1-1] if N <= kTailMaxN),
+// [cw] B2 values, [cw] Hermite abscissae, [cw] time rows, [cw] stencil starts, and for K
+// tangent directions [K N] tangent last rows + [K cw] tangent B2 values
+inline size_t rhs_lds_bytes(int N, int cw, int K) {
+    const int rows = N <= kTailMaxN ? 3 : 1;
+    return (size_t)(rows * N + cw) * 16 + (size_t)cw * (8 + 2 * 4) + (size_t)K * (N + cw) * 16;
+}
 
 // f(q) = lambda q^p + (1-lambda) q^p2 and its derivatives (math_ops.cpp:23-37)
 struct Model {
@@ -89,7 +100,31 @@ struct RhsArgs {
     int log_response;
     double* fields;   // non-null: materialise the 15 fields (debug)
     int fbuf;         // Floor/diff buffer of the current interpolation
+    int G;            // lanes per B2 entry (4, 8, 16 or 64; rhs_shape_valid)
 };
+
+// RHS row-kernel shape: G lanes per B2 entry (the reduction tree of the B2 stencil sums
+// depends on G, so results are reproducible for a fixed G); taps per lane per gather pass
+inline bool rhs_shape_valid(int G, int m) {
+    return (G == 4 && m <= 16) || (G == 8 && m <= 16) || (G == 16 && m <= 32) || G == 64;
+}
+inline int b2_taps_per_pass(int G, int m) {
+    const int nt = (m + G - 1) / G;
+    return nt < 1 ? 1 : (nt > 4 ? 4 : nt);
+}
+inline int rhs_default_G(int m) { return m <= 16 ? 8 : (m <= 32 ? 16 : 64); }
+// column chunks per row: cw = ceil(N / split) entries per block
+inline int rhs_default_split(int N) { return N >= 256 ? (N / 256 > 1 ? N / 256 : 1) : 1; }
+inline int rhs_max_split(int N) { return 2 * rhs_default_split(N); }
+
+// RHS shape of a grid class (N, m, log_response): the tuned entry of the process-wide
+// table (dmfe_autotune / dmfe_tuning_set) or the default; contexts and batches read it
+// when they are created.
+struct RhsShape {
+    int G, split;
+};
+RhsShape rhs_shape_lookup(int N, int m, int log_response);
+void rhs_shape_store(int N, int m, int log_response, RhsShape s);
 
 struct FinishArgs {
     int mode;         // 0: RHS only (into hK[stage]); 1: full RK stage
@@ -210,7 +245,7 @@ void batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk);
 void launch_batch_begin(InstDesc* D, int n, hipStream_t s);
 void launch_batch_end(InstDesc* D, int n, hipStream_t s);
 void launch_batch_stage(InstDesc* D, const int2* map, int nblocks, int n, int ept, hipStream_t s);
-void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_response, int G, size_t lds,
+void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_response, int G, int maxm, size_t lds,
                       hipStream_t s);
 void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t s);
 

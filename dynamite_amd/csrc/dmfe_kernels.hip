@@ -445,16 +445,16 @@ __device__ inline double2 stencil_last(const double* __restrict__ w, long long P
     }
     return make_double2(a, invalid ? lin : exp(ls + lc));
 }
+
 #pragma clang fp contract(on)
 
-// B2 (index_mat.cpp:18-91), generic form: groups of G lanes per entry, lane d holds
-// taps d, d+G, ...; gathers of one wave instruction touch 64/G entries x a few
-// contiguous cache lines; the m-tap sums are closed by group_sum and the time
-// Hermite is applied by the group's first lane.
+// B2 with the four slice sums (Y0, Y1, DY1, DY2) closed separately and the time Hermite
+// applied to the sums by the group's first lane: the log-response form, where QR is
+// interpolated in log space and is not linear in the slices.
 template <bool LOGR, int G>
-__device__ __forceinline__ void b2_groups(const Tables& T, const History& H, const Work& W, const RhsArgs& a, int j,
+__device__ __forceinline__ void b2_groups_sums(const Tables& T, const History& H, const Work& W, const RhsArgs& a, int j,
                                           int c0, int C, const int* b2ix, const int* b2s, const double* b2u,
-                                          double2* b2) {
+                                          double2* b2, const double2* tail /* [3][N] or nullptr */) {
     const int N = T.N, m = T.m, T1 = a.T1;
     constexpr int GPW = kWave / G;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
@@ -475,14 +475,45 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
         double acc8[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc8[q] = 0.0;
-        if (ev) {
-            for (int dd = d0; dd < m; dd += G) {
+        // taps dd and dd + G of a lane are loaded together (one memory round trip per
+        // pass instead of one per tap); accumulation order stays dd, dd + G, ...
+        if (ev && !b2int && tail) {  // last interval: the Hermite uses Y0, Y1, DY1 only
+            const double2 *L0 = tail + sst, *L1 = tail + N + sst, *L2 = tail + 2 * N + sst;
+            for (int dd = d0; dd < m; dd += 2 * G) {
+                const bool two = dd + G < m;
+                const double wd = wk[dd], we = two ? wk[dd + G] : 0.0;
+                const double2 v0 = L0[dd], v1 = L1[dd], g1 = L2[dd];
+                acc8[0] += wd * v0.x; acc8[1] += wd * v0.y;
+                acc8[2] += wd * v1.x; acc8[3] += wd * v1.y;
+                acc8[4] += wd * g1.x; acc8[5] += wd * g1.y;
+                if (two) {
+                    const double2 u0 = L0[dd + G], u1 = L1[dd + G], h1 = L2[dd + G];
+                    acc8[0] += we * u0.x; acc8[1] += we * u0.y;
+                    acc8[2] += we * u1.x; acc8[3] += we * u1.y;
+                    acc8[4] += we * h1.x; acc8[5] += we * h1.y;
+                }
+            }
+        } else if (ev) {
+            for (int dd = d0; dd < m; dd += 2 * G) {
+                const bool two = dd + G < m;
                 const double wd = wk[dd];
                 const double2 v0 = R0[dd], v1 = R1[dd], g1 = S1[dd], g2 = S2[dd];
+                double we = 0.0;
+                double2 u0 = make_double2(0.0, 0.0), u1 = u0, h1 = u0, h2 = u0;
+                if (two) {
+                    we = wk[dd + G];
+                    u0 = R0[dd + G]; u1 = R1[dd + G]; h1 = S1[dd + G]; h2 = S2[dd + G];
+                }
                 acc8[0] += wd * v0.x; acc8[1] += wd * v0.y;
                 acc8[2] += wd * v1.x; acc8[3] += wd * v1.y;
                 acc8[4] += wd * g1.x; acc8[5] += wd * g1.y;
                 acc8[6] += wd * g2.x; acc8[7] += wd * g2.y;
+                if (two) {
+                    acc8[0] += we * u0.x; acc8[1] += we * u0.y;
+                    acc8[2] += we * u1.x; acc8[3] += we * u1.y;
+                    acc8[4] += we * h1.x; acc8[5] += we * h1.y;
+                    acc8[6] += we * h2.x; acc8[7] += we * h2.y;
+                }
             }
         }
         const double y0k = group_sum<G>(acc8[0]), y0r = group_sum<G>(acc8[1]);
@@ -512,18 +543,120 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
     }
 }
 
+// Hermite coefficients of the four B2 slices (Y0, Y1, DY1, DY2) at abscissa u
+// (index_mat.cpp:45-70): cubic Hermite inside the history, the quadratic form in the last
+// interval (where DY2 does not enter).
+__device__ __forceinline__ void b2_hermite(double u, bool interior, double rr, double (&c)[4]) {
+    const double u2 = u * u, u3 = u2 * u;
+    if (interior) {
+        c[0] = 1 - 3 * u2 + 2 * u3;
+        c[1] = 3 * u2 - 2 * u3;
+        c[2] = u - 2 * u2 + u3;
+        c[3] = (-u2 + u3) / rr;
+    } else {
+        c[0] = 1 - u2;
+        c[1] = u2;
+        c[2] = u - u2;
+        c[3] = 0.0;
+    }
+}
+
+// one stencil tap of B2: the Hermite combination of the four slices, times the tap weight
+__device__ __forceinline__ void b2_tap(const double (&c)[4], double wd, double2 v0, double2 v1, double2 g1,
+                                       double2 g2, double& ak, double& ar) {
+    const double pk = fma(c[3], g2.x, fma(c[2], g1.x, fma(c[1], v1.x, c[0] * v0.x)));
+    const double pr = fma(c[3], g2.y, fma(c[2], g1.y, fma(c[1], v1.y, c[0] * v0.y)));
+    ak = fma(wd, pk, ak);
+    ar = fma(wd, pr, ar);
+}
+
+// B2 (index_mat.cpp:18-91), linear form: groups of G lanes per entry, lane d holds taps
+// d, d+G, ...; gathers of one wave instruction touch 64/G entries x a few contiguous
+// cache lines.  The interpolant is linear in the four slices, so every lane folds the
+// entry's time Hermite into its taps and a group closes only two sums (QK, QR).  Entries
+// in the last time interval (ix = T1-1: V[T1-2], V[T1-1], D[T1-1] only) read those rows
+// from the block's LDS copies `tail` = {V[T1-2], V[T1-1], D[T1-1]} when given.
+template <bool LOGR, int G, int NT>
+__device__ __forceinline__ void b2_groups(const Tables& T, const History& H, const Work& W, const RhsArgs& a, int j,
+                                          int c0, int C, const int* b2ix, const int* b2s, const double* b2u,
+                                          double2* b2, const double2* tail /* [3][N] or nullptr */) {
+    if constexpr (LOGR) {
+        b2_groups_sums<LOGR, G>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2, tail);
+        return;
+    }
+    const int N = T.N, m = T.m, T1 = a.T1;
+    constexpr int GPW = kWave / G;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int g = lane / G, d0 = lane % G;
+    constexpr int kWaves = kRowThreads / kWave;
+    for (int e0 = wid * GPW; e0 < C; e0 += GPW * kWaves) {
+        const int e = e0 + g;
+        const bool ev = e < C;
+        const int ee = ev ? e : 0;
+        const size_t ix = (size_t)b2ix[ee];
+        const int sst = b2s[ee];
+        const bool b2int = ix < (size_t)(T1 - 1);
+        const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
+        double c[4];
+        b2_hermite(b2u[ee], b2int, b2int ? H.rho[ix + 1] : 1.0, c);
+        double ak = 0.0, ar = 0.0;
+        // the NT taps dd, dd + G, ... of a lane are loaded in one pass (one memory round
+        // trip instead of one per tap); accumulation order stays dd, dd + G, ...
+        if (ev && !b2int && tail) {
+            const double2 *L0 = tail + sst, *L1 = tail + N + sst, *L2 = tail + 2 * N + sst;
+            for (int dd = d0; dd < m; dd += NT * G) {
+                double w[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) w[t] = dd + t * G < m ? wk[dd + t * G] : 0.0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int d = dd + t * G;
+                    if (d < m) {
+                        const double2 g1 = L2[d];
+                        b2_tap(c, w[t], L0[d], L1[d], g1, g1, ak, ar);
+                    }
+                }
+            }
+        } else if (ev) {
+            const double2* R0 = H.V + (ix - 1) * N + sst;
+            const double2* R1 = H.V + ix * N + sst;
+            const double2* S1 = H.D + ix * N + sst;
+            const double2* S2 = H.D + (b2int ? ix + 1 : ix) * N + sst;
+            for (int dd = d0; dd < m; dd += NT * G) {
+                double w[NT];
+                double2 v0[NT], v1[NT], g1[NT], g2[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int d = dd + t * G;
+                    w[t] = 0.0;
+                    v0[t] = v1[t] = g1[t] = g2[t] = make_double2(0.0, 0.0);
+                    if (d < m) {
+                        w[t] = wk[d];
+                        v0[t] = R0[d]; v1[t] = R1[d]; g1[t] = S1[d]; g2[t] = S2[d];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    if (dd + t * G < m) b2_tap(c, w[t], v0[t], v1[t], g1[t], g2[t], ak, ar);
+            }
+        }
+        const double qk = group_sum<G>(ak), qr = group_sum<G>(ar);
+        if (d0 == 0 && ev) b2[e] = make_double2(qk, qr);
+    }
+}
+
 // Row j, columns [c0, c1): the eight quadratures of QKstep/QRstep.
 //  phase 1 (B2, index_mat.cpp:18-91): one group of G lanes per target entry, lane d
 //          holding stencil tap d, so the four history-row gathers of a wave touch a
 //          few contiguous cache lines instead of one line per lane; the m-tap inner
-//          products are closed by an xor-shuffle reduction inside the group and the
-//          time Hermite is applied by the group's first lane.  Results go to LDS.
+//          products are closed by an xor-shuffle reduction inside the group.  Results
+//          go to LDS.
 //  phase 2 (lane per entry): A1/A2 stencils on the last row (LDS), the B1 time
 //          Hermite (index_vec.cpp:63-136), the self-energies and the weighted
 //          products, then one block reduction.
 // B2 of the primal history and of K tangent histories in one pass (linear form): the
-// weights and positions are loaded once per tap and applied to every history; each
-// history's sums and Hermite are the same operations as in b2_groups.
+// weights, positions and Hermite coefficients are computed once per tap and applied to
+// every history; the primal's operations are exactly those of b2_groups<false>.
 template <int G, int K>
 __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& H, const TanArgs& X, const RhsArgs& a,
                                                 int j, int c0, int C, const int* b2ix, const int* b2s,
@@ -551,42 +684,23 @@ __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& 
         const bool b2int = ix < (size_t)(T1 - 1);
         const size_t o0 = (ix - 1) * N + sst, o1 = ix * N + sst, o2 = (b2int ? ix + 1 : ix) * N + sst;
         const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
-        double acc8[1 + K][8];
+        double c[4];
+        b2_hermite(b2u[ee], b2int, b2int ? H.rho[ix + 1] : 1.0, c);
+        double ak[1 + K], ar[1 + K];
 #pragma unroll
-        for (int h = 0; h <= K; ++h)
-#pragma unroll
-            for (int q = 0; q < 8; ++q) acc8[h][q] = 0.0;
+        for (int h = 0; h <= K; ++h) ak[h] = ar[h] = 0.0;
         if (ev) {
             for (int dd = d0; dd < m; dd += G) {
                 const double wd = wk[dd];
 #pragma unroll
-                for (int h = 0; h <= K; ++h) {
-                    const double2 v0 = Vh[h][o0 + dd], v1 = Vh[h][o1 + dd], g1 = Dh[h][o1 + dd], g2 = Dh[h][o2 + dd];
-                    acc8[h][0] += wd * v0.x; acc8[h][1] += wd * v0.y;
-                    acc8[h][2] += wd * v1.x; acc8[h][3] += wd * v1.y;
-                    acc8[h][4] += wd * g1.x; acc8[h][5] += wd * g1.y;
-                    acc8[h][6] += wd * g2.x; acc8[h][7] += wd * g2.y;
-                }
+                for (int h = 0; h <= K; ++h)
+                    b2_tap(c, wd, Vh[h][o0 + dd], Vh[h][o1 + dd], Dh[h][o1 + dd], Dh[h][o2 + dd], ak[h], ar[h]);
             }
         }
-        const double u = b2u[ee], u2 = u * u, u3 = u2 * u;
-        const double rr = b2int ? H.rho[ix + 1] : 1.0;
 #pragma unroll
         for (int h = 0; h <= K; ++h) {
-            const double y0k = group_sum<G>(acc8[h][0]), y0r = group_sum<G>(acc8[h][1]);
-            const double y1k = group_sum<G>(acc8[h][2]), y1r = group_sum<G>(acc8[h][3]);
-            const double e1k = group_sum<G>(acc8[h][4]), e1r = group_sum<G>(acc8[h][5]);
-            const double e2k = group_sum<G>(acc8[h][6]), e2r = group_sum<G>(acc8[h][7]);
+            const double qk = group_sum<G>(ak[h]), qr = group_sum<G>(ar[h]);
             if (d0 == 0 && ev) {
-                double qk, qr;
-                if (b2int) {
-                    const double h0 = 1 - 3 * u2 + 2 * u3, h1 = u - 2 * u2 + u3, h2 = 3 * u2 - 2 * u3, h3 = -u2 + u3;
-                    qk = h0 * y0k + h1 * e1k + h2 * y1k + h3 * e2k / rr;
-                    qr = h0 * y0r + h1 * e1r + h2 * y1r + h3 * e2r / rr;
-                } else {
-                    qk = (1 - u2) * y0k + u2 * y1k + (u - u2) * e1k;
-                    qr = (1 - u2) * y0r + u2 * y1r + (u - u2) * e1r;
-                }
                 if (h == 0) b2[e] = make_double2(qk, qr);
                 else tb2[(size_t)(h - 1) * cw + e] = make_double2(qk, qr);
             }
@@ -627,7 +741,7 @@ __device__ inline void stencil_last_multi(const double* __restrict__ w, long lon
 // K > 0: also the tangent quadratures of K forward-mode directions (linear-response
 // form only): the tangent interpolants are the same stencils on the tangent histories,
 // the primal part is unchanged.
-template <bool LOGR, int G, int K = 0>
+template <bool LOGR, int G, int K = 0, int NT = 1>
 __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
                               int j, int chunk, double2* dyn /* [N] last row, [cw] (QKB2, QRB2), [cw] u, [cw] ix, s;
                                                                 K > 0: [K N] tangent last rows, [K cw] tangent B2 */,
@@ -638,13 +752,63 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     const int c0 = chunk * cw, c1 = min(N, c0 + cw);
     const int T1 = a.T1;
     const bool live = a.tl > 0;
-    double2* vlast = dyn;
-    double2* b2 = dyn + N;
-    double* b2u = reinterpret_cast<double*>(dyn + N + cw);
+    // LDS: rows {V[T1-2], V[T1-1], D[T1-1]} (only V[T1-1] when N > kTailMaxN), then the B2 arrays
+    const bool tail_rows = N <= kTailMaxN && T1 >= 2;
+    double2* rows = dyn;
+    double2* vlast = tail_rows ? dyn + N : dyn;
+    double2* b2 = dyn + (tail_rows ? 3 * N : N);
+    double* b2u = reinterpret_cast<double*>(b2 + cw);
     int* b2ix = reinterpret_cast<int*>(b2u + cw);
     int* b2s = b2ix + cw;
+    const int C = c1 - c0;
+    // the B2 operands of this thread's first entry and the scalars of phase 2 are
+    // requested first, so their latency overlaps the row copies below
+    const int e1 = threadIdx.x;
+    const long long k1 = (long long)j * N + c0 + e1;
+    double px1 = 0.0, pmax = 0.0;
+    int s1 = 0;
+    if (live) {
+        pmax = W.posB2[P - 1];
+        if (e1 < C) {
+            px1 = W.posB2[k1];
+            s1 = T.inds[2][k1];
+        }
+    }
+    const int F = W.Floor[(size_t)a.fbuf * N + j];
+    const double dB = W.diff[(size_t)a.fbuf * N + j];
+    const bool b1int = F < T1 - 1;
+    const double b1rho = b1int ? H.rho[F + 1] : 1.0;
+    // last rows into LDS, all loads of a thread in flight before its first store
     const double2* Vl = H.V + (size_t)(T1 - 1) * N;
-    for (int l = threadIdx.x; l < N; l += kRowThreads) vlast[l] = Vl[l];
+    const bool stage_tail = tail_rows && live;
+    const double2* Vp = H.V + (size_t)(stage_tail ? T1 - 2 : T1 - 1) * N;
+    const double2* Dl = H.D + (size_t)(T1 - 1) * N;
+    for (int base = 0; base < N; base += 4 * kRowThreads) {
+        double2 r0[4], r1[4], r2[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int l = base + q * kRowThreads + threadIdx.x;
+            r0[q] = r1[q] = r2[q] = make_double2(0.0, 0.0);
+            if (l < N) {
+                r0[q] = Vl[l];
+                if (stage_tail) {
+                    r1[q] = Vp[l];
+                    r2[q] = Dl[l];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int l = base + q * kRowThreads + threadIdx.x;
+            if (l < N) {
+                vlast[l] = r0[q];
+                if (stage_tail) {
+                    rows[l] = r1[q];
+                    rows[2 * N + l] = r2[q];
+                }
+            }
+        }
+    }
     double2* tvlast = reinterpret_cast<double2*>(b2s + cw);  // K > 0 only
     double2* tb2 = tvlast + (size_t)K * N;
     if constexpr (K > 0) {
@@ -655,36 +819,31 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
 
     // ---- phase 1: B2 ----------------------------------------------------------------
     const double c00 = H.V[0].x, c01 = H.V[0].y;
-    const int C = c1 - c0;
     if (live) {
         // 1a: per entry (lane per entry, coalesced): time row, stencil start, Hermite abscissa
-        size_t cap_hi = (size_t)(W.posB2[P - 1] - 0.5);
+        size_t cap_hi = (size_t)(pmax - 0.5);
         if (cap_hi > (size_t)(T1 - 1)) cap_hi = (size_t)(T1 - 1);  // memory guard
-        for (int e = threadIdx.x; e < C; e += kRowThreads) {
+        for (int e = e1; e < C; e += kRowThreads) {
             const long long k = (long long)j * N + c0 + e;
-            const double px = W.posB2[k];
+            const double px = e == e1 ? px1 : W.posB2[k];
             size_t ix = (size_t)px;
             if (cap_hi < ix) ix = cap_hi;
             if (ix < 1) ix = 1;
             b2ix[e] = (int)ix;
-            b2s[e] = T.inds[2][k];
+            b2s[e] = e == e1 ? s1 : T.inds[2][k];
             b2u[e] = px - ix;
         }
         __syncthreads();
         // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
         // instruction touch 64/G entries x a few contiguous cache lines
-        if constexpr (K == 0) b2_groups<LOGR, G>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2);
+        if constexpr (K == 0) b2_groups<LOGR, G, NT>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2, tail_rows ? rows : nullptr);
         else b2_groups_multi<G, K>(T, H, *X, a, j, c0, C, b2ix, b2s, b2u, b2, tb2, cw);
     }
     __syncthreads();
 
     // ---- phase 2: A1, A2, B1, self-energies, quadrature products -------------------------
-    const int F = W.Floor[(size_t)a.fbuf * N + j];
-    const double dB = W.diff[(size_t)a.fbuf * N + j];
     const double in3 = dB * dB, in4 = in3 * dB;
-    const bool b1int = F < T1 - 1;
     const double b1a = 1 - 3 * in3 - 2 * in4, b1b = 3 * in3 + 2 * in4, b1c = dB + 2 * in3 + in4, b1d = in3 + in4;
-    const double b1rho = b1int ? H.rho[F + 1] : 1.0;
     const double2* VF0 = H.V + (size_t)(F - 1) * N;
     const double2* VF1 = H.V + (size_t)F * N;
     const double2* DF1 = H.D + (size_t)F * N;
@@ -705,6 +864,8 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
             qkA1 = qkA2 = qkB1 = qkB2 = c00;
             qrA1 = qrA2 = qrB1 = qrB2 = c01;
         } else {
+            // this entry's B1 rows are requested before the stencil weights
+            const double2 y0 = VF0[i], y1 = VF1[i], dy1 = DF1[i], dy2 = DF2[i];
             double2 a1, a2;
             if constexpr (K > 0) {
                 const bool dense = W.fA[0] != nullptr;
@@ -722,9 +883,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
                 a2 = stencil_last<LOGR>(T.w[1], P, k, m, T.inds[1][k], vlast);
             }
             qkA1 = a1.x; qrA1 = a1.y; qkA2 = a2.x; qrA2 = a2.y;
-            const double2 y0 = VF0[i], y1 = VF1[i], dy1 = DF1[i];
             if (b1int) {
-                const double2 dy2 = DF2[i];
                 qkB1 = b1a * y0.x + b1b * y1.x - b1c * dy1.x - b1d * dy2.x / b1rho;
                 if (!LOGR) {
                     qrB1 = b1a * y0.y + b1b * y1.y - b1c * dy1.y - b1d * dy2.y / b1rho;
@@ -830,12 +989,12 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     }
 }
 
-template <bool LOGR, int G>
+template <bool LOGR, int G, int NT>
 __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History H, Work W, Model M, RhsArgs a) {
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    rhs_rows_body<LOGR, G>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red);
+    rhs_rows_body<LOGR, G, 0, NT>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red);
 }
 
 // the RHS with the tangent quadratures of K sensitivity directions (linear-response form)
@@ -1257,7 +1416,7 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(const InstDe
 
 // RHS launch n: map[b] = (instance, row j, chunk, -); blocks of one (j, chunk) for all
 // instances sharing a grid are adjacent, so their stencil weights are fetched once per XCD.
-template <bool LOGR, int G>
+template <bool LOGR, int G, int NT>
 __global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* __restrict__ D,
                                                                 const int4* __restrict__ map, int n) {
     extern __shared__ double2 dyn[];
@@ -1274,7 +1433,7 @@ __global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* 
     a.log_response = LOGR;
     a.fields = nullptr;
     a.fbuf = (c.fc0 + searches_before(c, n + 1)) & 1;
-    rhs_rows_body<LOGR, G>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red);
+    rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red);
 }
 
 // Observables of the instances that just stepped: (t_prop, E(t_prop), QK(t,0)) per row
@@ -1609,19 +1768,19 @@ void launch_tan_rhs(const Tables& T, const History& H, const Work& W, const Mode
                     const TanArgs& X, hipStream_t s) {
     const dim3 grid((unsigned)(T.N * a.split));
     const int cw = (T.N + a.split - 1) / a.split;
-    const size_t lds = (size_t)(T.N + cw) * sizeof(double2) + (size_t)cw * (sizeof(double) + 2 * sizeof(int)) +
-                       (size_t)X.K * (T.N + cw) * sizeof(double2);
+    const size_t lds = rhs_lds_bytes(T.N, cw, X.K);
 #define DMFE_TR(GG, KK) \
     hipLaunchKernelGGL((rhs_rows_tan_kernel<GG, KK>), grid, dim3(kRowThreads), lds, s, T, H, W, M, a, X)
-    if (X.K == 1) {
-        if (T.m <= 16) DMFE_TR(8, 1);
-        else if (T.m <= 32) DMFE_TR(16, 1);
-        else DMFE_TR(64, 1);
-    } else {
-        if (T.m <= 16) DMFE_TR(8, 2);
-        else if (T.m <= 32) DMFE_TR(16, 2);
-        else DMFE_TR(64, 2);
+#define DMFE_TRG(KK)                         \
+    switch (a.G) {                           \
+        case 4: DMFE_TR(4, KK); break;       \
+        case 8: DMFE_TR(8, KK); break;       \
+        case 16: DMFE_TR(16, KK); break;     \
+        default: DMFE_TR(64, KK); break;     \
     }
+    if (X.K == 1) DMFE_TRG(1)
+    else DMFE_TRG(2)
+#undef DMFE_TRG
 #undef DMFE_TR
 }
 
@@ -1675,21 +1834,36 @@ void launch_stage(const Tables& T, const History& H, const Work& W, const Model&
     else hipLaunchKernelGGL(stage_kernel<8>, dim3(nb), dim3(kStageThreads), 0, s, T, H, W, M, S, F, flags);
 }
 
+// (G, NT) shapes of the RHS row kernels: G lanes per B2 entry, NT taps per lane per
+// gather pass; the log-response form closes its sums per slice and ignores NT.
+#define DMFE_SHAPES(LOGR, G, NT, LAUNCH)                 \
+    if (LOGR) {                                          \
+        switch (G) {                                     \
+            case 4: LAUNCH(true, 4, 1); break;           \
+            case 8: LAUNCH(true, 8, 1); break;           \
+            case 16: LAUNCH(true, 16, 1); break;         \
+            default: LAUNCH(true, 64, 1); break;         \
+        }                                                \
+    } else {                                             \
+        switch (G) {                                     \
+            case 4:                                      \
+                if (NT <= 3) LAUNCH(false, 4, 3);        \
+                else LAUNCH(false, 4, 4);                \
+                break;                                   \
+            case 8: LAUNCH(false, 8, 2); break;          \
+            case 16: LAUNCH(false, 16, 2); break;        \
+            default: LAUNCH(false, 64, 1); break;        \
+        }                                                \
+    }
+
 void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
                      hipStream_t s) {
     const dim3 grid((unsigned)(T.N * a.split));
     const int cw = (T.N + a.split - 1) / a.split;
-    const size_t lds = (size_t)(T.N + cw) * sizeof(double2) + (size_t)cw * (sizeof(double) + 2 * sizeof(int));
-#define DMFE_RHS(LOG, G) hipLaunchKernelGGL((rhs_rows_kernel<LOG, G>), grid, dim3(kRowThreads), lds, s, T, H, W, M, a)
-    if (a.log_response) {
-        if (T.m <= 16) DMFE_RHS(true, 8);
-        else if (T.m <= 32) DMFE_RHS(true, 16);
-        else DMFE_RHS(true, 64);
-    } else {
-        if (T.m <= 16) DMFE_RHS(false, 8);
-        else if (T.m <= 32) DMFE_RHS(false, 16);
-        else DMFE_RHS(false, 64);
-    }
+    const size_t lds = rhs_lds_bytes(T.N, cw, 0);
+#define DMFE_RHS(LOG, G, NT) \
+    hipLaunchKernelGGL((rhs_rows_kernel<LOG, G, NT>), grid, dim3(kRowThreads), lds, s, T, H, W, M, a)
+    DMFE_SHAPES(a.log_response, a.G, b2_taps_per_pass(a.G, T.m), DMFE_RHS)
 #undef DMFE_RHS
 }
 
@@ -1739,19 +1913,11 @@ void launch_batch_stage(InstDesc* D, const int2* map, int nblocks, int n, int ep
     else hipLaunchKernelGGL(batch_stage_kernel<8>, dim3(nblocks), dim3(kStageThreads), 0, s, D, map, n);
 }
 
-void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_response, int G, size_t lds,
-                      hipStream_t s) {
-#define DMFE_BRHS(LOG, GG) \
-    hipLaunchKernelGGL((batch_rhs_kernel<LOG, GG>), dim3(nblocks), dim3(kRowThreads), lds, s, D, map, n)
-    if (log_response) {
-        if (G <= 8) DMFE_BRHS(true, 8);
-        else if (G <= 16) DMFE_BRHS(true, 16);
-        else DMFE_BRHS(true, 64);
-    } else {
-        if (G <= 8) DMFE_BRHS(false, 8);
-        else if (G <= 16) DMFE_BRHS(false, 16);
-        else DMFE_BRHS(false, 64);
-    }
+void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_response, int G, int maxm,
+                      size_t lds, hipStream_t s) {
+#define DMFE_BRHS(LOG, GG, NT) \
+    hipLaunchKernelGGL((batch_rhs_kernel<LOG, GG, NT>), dim3(nblocks), dim3(kRowThreads), lds, s, D, map, n)
+    DMFE_SHAPES(log_response, G, b2_taps_per_pass(G, maxm), DMFE_BRHS)
 #undef DMFE_BRHS
 }
 

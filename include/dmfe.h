@@ -253,6 +253,32 @@ int dmfe_dense_apply(dmfe_ctx* ctx, int map, int B, const double* X, double* Y, 
  * *ms_out = average per launch in ms. */
 int dmfe_time_rhs(dmfe_ctx* ctx, int reps, double* ms_out);
 
+/* RHS kernel shape (no counterpart in the reference, whose CUDA kernels use fixed
+ * launch shapes): G lanes per B2 target entry (4 or 8 for m <= 16, 16 for m <= 32, 64
+ * for any m) and `split` column chunks per row (1 .. 2*max(1, N/256)).  The B2 stencil
+ * sums are reduced over G lanes and the quadratures over split chunks, so results are
+ * reproducible for a fixed shape and differ in the last bits between shapes. */
+typedef struct dmfe_rhs_shape {
+    int32_t G;
+    int32_t split;
+} dmfe_rhs_shape;
+
+int dmfe_get_rhs_shape(const dmfe_ctx* ctx, dmfe_rhs_shape* out);
+/* Sets the context's shape (before dmfe_tangent_init). */
+int dmfe_set_rhs_shape(dmfe_ctx* ctx, const dmfe_rhs_shape* shape);
+/* Process-wide shape table per grid class (N, m, log_response): contexts and batches take
+ * their shape from it when created (default G = 8/16/64 by m, split = max(1, N/256)).
+ * A batch runs one G for all instances: that of its first instance's class. */
+int dmfe_tuning_get(int64_t N, int m, int log_response, dmfe_rhs_shape* out);
+int dmfe_tuning_set(int64_t N, int m, int log_response, const dmfe_rhs_shape* shape);
+/* Autotuner: times `reps` back-to-back RHS launches for every valid (G, split) on the
+ * context's current interpolated history (t > 0; best of two sweeps), keeps the fastest
+ * on the context and, if `install`, stores it in the table for the context's class.
+ * tried/tried_ms (may be NULL) receive up to max_tried candidates and their average
+ * launch times in ms; *n_tried = number of candidates. */
+int dmfe_autotune(dmfe_ctx* ctx, int reps, int install, dmfe_rhs_shape* best, dmfe_rhs_shape* tried,
+                  double* tried_ms, int max_tried, int* n_tried);
+
 /* Measurement utility (no counterpart in the reference): `reps` launches of a
  * plain device-to-device copy of `bytes` bytes (hbm_copy_kernel), used to
  * calibrate HBM byte counters; *ms_out = average launch time in ms. */
@@ -316,6 +342,16 @@ int dmfe_batch_get_state(dmfe_batch* b, int i, double* t1grid, double* QK, doubl
  * (cap x 4 doubles). */
 int dmfe_batch_record(dmfe_batch* b, int cap);
 int dmfe_batch_records(dmfe_batch* b, int i, double* out);
+
+/* Lane-group width G of the batch RHS kernel (one for all instances; see
+ * dmfe_rhs_shape; each instance keeps the split of its class in the shape table).
+ * dmfe_batch_autotune times `reps` launches of the batch RHS on the current states for
+ * every valid G (best of two sweeps, after at least one batch step), keeps the fastest
+ * and returns it with the candidates and their average launch times (ms). */
+int dmfe_batch_get_lanes(const dmfe_batch* b, int* G);
+int dmfe_batch_set_lanes(dmfe_batch* b, int G);
+int dmfe_batch_autotune(dmfe_batch* b, int reps, int* G_best, int32_t* tried_G, double* tried_ms, int max_tried,
+                        int* n_tried);
 
 #ifdef __cplusplus
 }
