@@ -1319,48 +1319,57 @@ __device__ inline void make_finish_dev(const Ctl& c, const InstDesc& d, int m, F
     f.fbuf = (c.fc0 + searches_before(c, m + 1)) & 1;
 }
 
-// begin: loop condition, stage times gt[] and the error's time term (runge_kutta.cpp:167,201,207-211,234-238)
-__global__ void batch_begin_kernel(InstDesc* D, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// begin: loop condition, stage times gt[] and the error's time term (runge_kutta.cpp:167,201,207-211,234-238).
+// One block per instance: lane 0 runs the scalar control, then lanes 0..S build the
+// search arguments of stages 0..S and lanes 16..16+S-1 the finish arguments.
+constexpr int kBeginThreads = 32;
+__global__ __launch_bounds__(kBeginThreads) void batch_begin_kernel(InstDesc* D, int n) {
+    const int i = blockIdx.x;
     if (i >= n) return;
     Ctl& c = *D[i].ctl;
-    c.active = 0;
-    if (c.status != kRunning) return;
-    const double eps = 2.220446049250313e-16;
-    const double tol = eps * fmax(fabs(c.dt), fabs(c.dt_min));
-    if (!(c.t_prop < c.tmax && c.loop < c.max_loop && c.dt >= c.dt_min - tol)) {
-        c.status = kDone;
-        return;
+    __shared__ int go;
+    if (threadIdx.x == 0) {
+        go = 0;
+        c.active = 0;
+        const double eps = 2.220446049250313e-16;
+        const double tol = eps * fmax(fabs(c.dt), fabs(c.dt_min));
+        if (c.status != kRunning) {
+        } else if (!(c.t_prop < c.tmax && c.loop < c.max_loop && c.dt >= c.dt_min - tol)) {
+            c.status = kDone;
+        } else if (c.T1 + 1 > c.cap) {
+            c.status = kNeedRows;
+        } else {
+            const double* t1 = D[i].H.t1;
+            c.T1s = c.T1;
+            c.gt[0] = t1[c.T1 - 1];
+            c.tprev = c.T1 >= 2 ? t1[c.T1 - 2] : 0.0;
+            c.t_prop = c.gt[0] + c.dt;
+            c.delta_old = c.delta;
+            const int t = c.method == DMFE_RK54 ? 0 : 1;
+            const int S = t == 0 ? 7 : 10;
+            c.S = S;
+            const double dt = c.dt;
+            c.gt[1] = c.gt[0] + dt * cA[t][1][0] * 1.0;
+            for (int m = 1; m < S; ++m) {
+                const bool last = (m == S - 1);
+                double g = c.gt[0];
+                const int nt = last ? S : m + 1;
+                for (int j = 0; j < nt; ++j) g += dt * (last ? cB[t][j] : cA[t][m + 1][j]) * 1.0;
+                c.gt[m + 1] = g;
+            }
+            double gte = c.gt[0];
+            for (int j = 0; j < S; ++j) gte += dt * cB2[t][j] * 1.0;
+            c.gt_err = fabs(c.gt[S] - gte);
+            c.fc0 = c.fcur;
+            go = 1;
+        }
     }
-    if (c.T1 + 1 > c.cap) {
-        c.status = kNeedRows;
-        return;
-    }
-    const double* t1 = D[i].H.t1;
-    c.T1s = c.T1;
-    c.gt[0] = t1[c.T1 - 1];
-    c.tprev = c.T1 >= 2 ? t1[c.T1 - 2] : 0.0;
-    c.t_prop = c.gt[0] + c.dt;
-    c.delta_old = c.delta;
-    const int t = c.method == DMFE_RK54 ? 0 : 1;
-    const int S = t == 0 ? 7 : 10;
-    c.S = S;
-    const double dt = c.dt;
-    c.gt[1] = c.gt[0] + dt * cA[t][1][0] * 1.0;
-    for (int m = 1; m < S; ++m) {
-        const bool last = (m == S - 1);
-        double g = c.gt[0];
-        const int nt = last ? S : m + 1;
-        for (int j = 0; j < nt; ++j) g += dt * (last ? cB[t][j] : cA[t][m + 1][j]) * 1.0;
-        c.gt[m + 1] = g;
-    }
-    double gte = c.gt[0];
-    for (int j = 0; j < S; ++j) gte += dt * cB2[t][j] * 1.0;
-    c.gt_err = fabs(c.gt[S] - gte);
-    c.fc0 = c.fcur;
-    for (int k = 0; k <= S; ++k) make_search_dev(c, D[i], k, c.srch[k]);
-    for (int k = 0; k < S; ++k) make_finish_dev(c, D[i], k, c.fin[k]);
-    c.active = 1;
+    __syncthreads();
+    if (!go) return;
+    const int k = threadIdx.x;
+    if (k <= c.S) make_search_dev(c, D[i], k, c.srch[k]);
+    else if (k >= 16 && k - 16 < c.S) make_finish_dev(c, D[i], k - 16, c.fin[k - 16]);
+    if (k == 0) c.active = 1;
 }
 
 // end: delta, loop, maintenance cadence and the adaptive dt rule (simulation_runner.cpp:144-278)
@@ -1899,7 +1908,7 @@ void batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk) {
 }
 
 void launch_batch_begin(InstDesc* D, int n, hipStream_t s) {
-    hipLaunchKernelGGL(batch_begin_kernel, dim3((n + 63) / 64), dim3(64), 0, s, D, n);
+    hipLaunchKernelGGL(batch_begin_kernel, dim3(n), dim3(kBeginThreads), 0, s, D, n);
 }
 
 void launch_batch_end(InstDesc* D, int n, hipStream_t s) {
