@@ -54,10 +54,14 @@ PARAMS = dict(p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.6)
 
 
 def gpu_run(g, prep, steps, warmup):
+    """The plain quench at L=512: `prep` steps from the quench, then `warmup` + `steps`
+    timed steps of dmfe_run (the main loop with the step control on the device, replayed
+    hipGraphs); then 50 steps of the host-driven loop (dmfe_loop_iteration) with per-launch
+    events for the in-step kernel averages, and back-to-back RHS launches."""
     import dynamite_amd as dm
     prm = dm.default_params(p=PARAMS["p"], p2=PARAMS["p2"], lambda_=PARAMS["lam"], T0=PARAMS["T0"],
                             Gamma=PARAMS["Gamma"])
-    ctx = dm.Context(g, prm, rows_hint=prep + steps + warmup + 64)
+    ctx = dm.Context(g, prm, rows_hint=prep + steps + warmup + 128)
     ctx.init_quench()
     t0 = time.perf_counter()
     n, _ = ctx.run(1e300, prep)
@@ -70,16 +74,25 @@ def gpu_run(g, prep, steps, warmup):
     tune = dict(shape={"G": best[0], "split": best[1]},
                 candidates_us={f"G{G}_split{sp}": round(1e3 * ms, 2) for (G, sp), ms in table.items()})
     log(f"[bench] autotune: best G={best[0]} split={best[1]}; {tune['candidates_us']}")
-    for _ in range(warmup):
-        ctx.loop_iteration()
+    loop = ctx.get_control()["loop"]
+    ctx.run(1e300, loop + warmup)
     ctx.synchronize()
     state, control = ctx.get_state(), ctx.get_control()
-    dm.profile(ctx, True)
+    loop = control["loop"]
     t0 = time.perf_counter()
-    for _ in range(steps):
-        ctx.loop_iteration()
+    n, _ = ctx.run(1e300, loop + steps)
     ctx.synchronize()
     el = time.perf_counter() - t0
+    if n != steps:
+        raise RuntimeError(f"dmfe_run stopped after {n} of {steps} steps")
+    # host-driven loop on the continuing state: per-launch events inside the steps
+    dm.profile(ctx, True)
+    host_steps = 50
+    t0 = time.perf_counter()
+    for _ in range(host_steps):
+        ctx.loop_iteration()
+    ctx.synchronize()
+    el_host = time.perf_counter() - t0
     prof = {k: dm.profile_read(ctx, k) for k in (0, 1, 2)}
     dm.profile(ctx, False)
     # the RHS kernel alone, back to back on the final state (per-launch events inside
@@ -87,7 +100,8 @@ def gpu_run(g, prep, steps, warmup):
     rhs_b2b_ms = dm.time_rhs(ctx, 200)
     T1 = ctx.T1
     ctx.close()
-    return el, prof, state, control, T1, rhs_b2b_ms, tune
+    host = dict(steps=host_steps, value=host_steps / el_host, ms_per_step=1e3 * el_host / host_steps)
+    return el, prof, state, control, T1, rhs_b2b_ms, tune, host
 
 
 def cpu_baseline(g, state, control, budget_s=15.0):
@@ -186,7 +200,7 @@ def dense_microbench(N=512, order=9, B=4, reps=10):
                 fp64_mfma_peak_TFLOPs=78.6, dense_rhs_us=rhs_ms * 1e3)
 
 
-def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6):
+def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
     """Gradients of E and QK(t,0) at t_final w.r.t. (Gamma, lambda) for p=3+4, lambda=0.5,
     Gamma=0.7 on an L=256 poly-9 grid (forward-mode tangents carried with the run): the
     cost against the same run without tangents, and, on a shorter horizon, central finite
@@ -239,12 +253,32 @@ def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6):
         Jfd[:, col] = (replay(0.7 + dg, 0.5 + dl) - replay(0.7 - dg, 0.5 - dl)) / (2 * h)
     err = float(np.max(np.abs(J - Jfd)) / np.max(np.abs(Jfd)))
     log(f"[bench] sensitivities vs finite differences at t={t:.4g} ({len(seq)} steps): max rel err {err:.2e}")
+    # 16 (Gamma, lambda) points (seeded), one after another and all at once
+    rng = np.random.default_rng(BATCH_SEED + 16)
+    pts = [(float(G), float(lam)) for G, lam in zip(rng.uniform(0.3, 1.0, 16), rng.uniform(0.0, 1.0, 16))]
+    pl = [prm(G, lam) for G, lam in pts]
+    t0 = time.perf_counter()
+    seq_grads = []
+    for p in pl:
+        c = dm.Context(g, p, rows_hint=1024)
+        seq_grads.append(c.gradients(many_tmax, 10**12)[0])
+        c.close()
+    el_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grads_m, tf_m, st_m = dm.gradients_many([g] * len(pl), pl, many_tmax, 10**12)
+    el_many = time.perf_counter() - t0
+    same = bool(np.array_equal(np.array(seq_grads), grads_m))
+    log(f"[bench] 16 points to t={many_tmax:g} ({int(st_m.sum())} steps): one after another {el_seq:.2f}s, "
+        f"concurrent {el_many:.2f}s ({el_seq / el_many:.2f}x), identical={same}")
+    many = dict(points=len(pl), t_final=many_tmax, total_steps=int(st_m.sum()), sequential_s=el_seq,
+                concurrent_s=el_many, speedup=el_seq / el_many, identical=same,
+                workload="Gamma ~ U[0.3,1], lambda ~ U[0,1] (seeded), same grid and model")
     return dict(workload=f"L=256 poly-9 grid (spline 5, Tmax 1e5), p=3+4, lambda=0.5, Gamma=0.7, quench to "
                          f"t_final={t_final:.4g} ({n_grad} adaptive RK steps)",
                 grad={"dE_dGamma": grad[0], "dE_dlambda": grad[1], "dQK0_dGamma": grad[2], "dQK0_dlambda": grad[3]},
                 mode="forward (tangent-linear), 2 directions, dt sequence frozen",
                 forward_s=el_fwd, with_tangents_s=el_grad, cost_ratio=el_grad / el_fwd,
-                fd_check={"t_final": t, "steps": len(seq), "h": h, "max_rel_err": err})
+                fd_check={"t_final": t, "steps": len(seq), "h": h, "max_rel_err": err}, batched=many)
 
 
 BATCH_SEED = 20261203
@@ -297,33 +331,44 @@ def batch_run(prep, steps, looped_steps, chunk=16):
     batch_rate = inst_steps / el
     log(f"[bench] batch: {nb} batch steps, {inst_steps} instance-steps in {el:.3f}s -> {batch_rate:.0f}/s "
         f"({1e3 * el / max(nb, 1):.2f} ms per batch step)")
-    # looped plain path on the same states; the first context of each grid size is
-    # autotuned and installs its shape for the contexts created after it
+    # looped plain path on the same states, one context per instance advanced on its own
+    # (one instance per launch): with dmfe_run (device-side step control, the faster
+    # plain path) and with the host-driven dmfe_loop_iteration; the first context of each
+    # grid size is autotuned and installs its shape for the contexts created after it
     ctxs, tuned = [], {}
     for i in range(n):
         st, c = states[i]
-        ctx = dm.Context(gl[i], pl[i], rows_hint=len(st["t1grid"]) + looped_steps + 64)
+        ctx = dm.Context(gl[i], pl[i], rows_hint=len(st["t1grid"]) + 3 * looped_steps + 64)
         ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
         ctx.set_control(c["delta_t"], c["delta"], c["delta_old"], c["loop"], c["method"])
         ctx.interpolate(dm.POS_BSEARCH)
         if gl[i].N not in tuned:
             tuned[gl[i].N] = dm.autotune(ctx, reps=10)[0]
-        ctx.loop_iteration()  # warm-up
+        ctx.run(1e300, c["loop"] + looped_steps)  # warm-up, graph capture
         ctx.synchronize()
         ctxs.append(ctx)
     log(f"[bench] looped path shapes (G, split) by L: {tuned}")
     t0 = time.perf_counter()
     for ctx in ctxs:
+        ctx.run(1e300, ctx.get_control()["loop"] + looped_steps)
+        ctx.synchronize()
+    el_l = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for ctx in ctxs:
         for _ in range(looped_steps):
             ctx.loop_iteration()
         ctx.synchronize()
-    el_l = time.perf_counter() - t0
+    el_h = time.perf_counter() - t0
     for ctx in ctxs:
         ctx.close()
     looped_rate = n * looped_steps / el_l
-    log(f"[bench] looped plain path: {n} x {looped_steps} steps in {el_l:.3f}s -> {looped_rate:.0f}/s")
-    return dict(instances=n, value=batch_rate, unit="instance-steps/s", looped_value=looped_rate,
-                ratio=batch_rate / looped_rate, batch_steps=nb, instance_steps=inst_steps,
+    host_rate = n * looped_steps / el_h
+    log(f"[bench] looped plain path: {n} x {looped_steps} steps: dmfe_run {el_l:.3f}s -> {looped_rate:.0f}/s, "
+        f"host-driven loop {el_h:.3f}s -> {host_rate:.0f}/s")
+    best = max(looped_rate, host_rate)
+    return dict(instances=n, value=batch_rate, unit="instance-steps/s", looped_value=best,
+                ratio=batch_rate / best, looped_dmfe_run_value=looped_rate, looped_host_driven_value=host_rate,
+                looped_steps=looped_steps, batch_steps=nb, instance_steps=inst_steps,
                 ms_per_batch_step=1e3 * el / max(nb, 1), prep_steps=prep, graph_chunk=chunk,
                 autotune={"batch_G": G_b, "batch_rhs_ms": {f"G{G}": round(ms, 4) for G, ms in table_b.items()},
                           "looped_shapes": {f"L{L}": {"G": sh[0], "split": sh[1]} for L, sh in tuned.items()}},
@@ -360,8 +405,9 @@ def main():
     ap.add_argument("--no-sensitivity", action="store_true")
     ap.add_argument("--batch-prep", type=int, default=1000)
     ap.add_argument("--batch-steps", type=int, default=64)
-    ap.add_argument("--looped-steps", type=int, default=10)
+    ap.add_argument("--looped-steps", type=int, default=48)
     ap.add_argument("--batch-only", action="store_true", help="only the batch-64 measurement")
+    ap.add_argument("--sensitivity-only", action="store_true", help="only the sensitivity measurement")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain HBM copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args()
@@ -376,6 +422,9 @@ def main():
         print(f"bench.py: this build runs on one GPU only (got --gpus {a.gpus}); "
               "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
         sys.exit(2)
+    if a.sensitivity_only:
+        print(json.dumps({"sensitivities": sensitivity_run()}), flush=True)
+        return
     if a.batch_only:
         out = batch_run(a.batch_prep, a.batch_steps, a.looped_steps)
         print(json.dumps({"batch": out}), flush=True)
@@ -383,7 +432,7 @@ def main():
     N = 512
     g = make_grid(N)
     log(f"[bench] GPU: {a.prep} prep + {a.warmup} warmup + {a.steps} timed steps")
-    el, prof, state, control, T1, rhs_b2b_ms, tune = gpu_run(g, a.prep, a.steps, a.warmup)
+    el, prof, state, control, T1, rhs_b2b_ms, tune, host = gpu_run(g, a.prep, a.steps, a.warmup)
     ms_step = 1e3 * el / a.steps
     rhs_ms, rhs_n = prof[1]
     avg_rhs_in_steps_s = rhs_ms / 1e3 / max(rhs_n, 1)
@@ -410,7 +459,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "plain quench, L=512 poly-9 grid (spline 5, Tmax 1e5), p=3 q=4 lambda=0.5 "
-                               "Gamma=0.6 T0=inf; adaptive RK steps on the live state after the prep steps",
+                               "Gamma=0.6 T0=inf; adaptive RK steps (dmfe_run) on the live state after the prep steps",
                    "L": N, "stencil_width": g.m, "prep_steps": a.prep, "T1_end": T1,
                    "method": "RK54" if control["method"] == 1 else "SSPRK104", "delta_t": control["delta_t"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -419,6 +468,7 @@ def main():
                      "avg_launch_us": avg_rhs_s * 1e6, "avg_launch_us_in_steps": avg_rhs_in_steps_s * 1e6,
                      "launches_in_steps": rhs_n},
         "autotune": tune,
+        "host_driven_loop": host,
     }
     out["kernel_microbench"] = kernel_microbench(g)
     log(f"[bench] microbench: {out['kernel_microbench']}")

@@ -351,6 +351,22 @@ def time_rhs(ctx, reps=100):
     return ms.value
 
 
+def gradients_many(grids, params, tmax, max_loop, rows_hint=1024):
+    """dmfe_gradients for several (grid, params) points concurrently (one context, stream
+    and host thread each).  Returns (grads [n, 4], t_final [n], steps [n])."""
+    n = len(grids)
+    if n != len(params) or n == 0:
+        raise ValueError("gradients_many: need one grid and one params per point")
+    gs = (Grid * n)(*[g.as_struct() for g in grids])
+    ps = (Params * n)(*params)
+    grads = np.zeros((n, 4))
+    tf = np.zeros(n)
+    st = np.zeros(n, dtype=np.int64)
+    _check(lib().dmfe_gradients_many(C.c_int(n), gs, ps, C.c_double(tmax), C.c_int64(max_loop), C.c_int64(rows_hint),
+                                     _ptr(grads), _ptr(tf), _ptr(st)), "dmfe_gradients_many")
+    return grads, tf, st
+
+
 def rhs_shape(ctx):
     """(G, split) of the context's RHS kernel."""
     sh = RhsShape()

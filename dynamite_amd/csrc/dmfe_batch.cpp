@@ -98,6 +98,7 @@ struct dmfe_batch {
     int graph_steps[2] = {0, 0};
     bool graph_obs[2] = {false, false};
     int64_t steps_run = 0;
+    bool borrowed = false;  // device_loop_run: tables, instance buffers and stream belong to a context
 };
 
 namespace {
@@ -325,6 +326,12 @@ void free_batch(dmfe_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
+    if (b->borrowed) {
+        (void)hipFree(b->d_desc); (void)hipFree(b->d_ctl); (void)hipFree(b->d_smap); (void)hipFree(b->d_rmap);
+        (void)hipFree(b->d_obs);
+        delete b;
+        return;
+    }
     for (GridDev& g : b->grids) {
         (void)hipFree(g.theta); (void)hipFree(g.integ); (void)hipFree(g.phi2);
         for (int f = 0; f < 3; ++f) { (void)hipFree(g.inds[f]); (void)hipFree(g.w[f]); }
@@ -390,6 +397,152 @@ int alloc_work(Work& W, int N, int split) {
 }  // namespace
 
 // ============================== C ABI ==============================================
+
+// ---- one context's main loop on the device-side control ---------------------------------
+namespace dmfe {
+
+namespace {
+constexpr int kLoopChunk = 16;  // steps per graph replay of a context's device loop
+
+// a borrowing one-instance batch for the context of q (tables, work, stream are q's)
+int device_loop_open(const DeviceLoop& q, dmfe_batch** out) {
+    const int N = q.T.N;
+    dmfe_batch* b = new dmfe_batch();
+    b->borrowed = true;
+    b->stream = q.stream;
+    (void)hipGetDevice(&b->device);
+    batch_upload_tableaux(dp54(), ssprk104());
+    b->log_response = q.prm.log_response;
+    GridDev gd;
+    gd.N = N; gd.m = q.T.m; gd.P = q.T.P;
+    // borrowed, never freed here (free_batch skips the tables of a borrowing batch)
+    gd.theta = const_cast<double*>(q.T.theta);
+    gd.integ = const_cast<double*>(q.T.integ);
+    gd.phi2 = const_cast<double*>(q.T.phi2);
+    for (int f = 0; f < 3; ++f) {
+        gd.inds[f] = const_cast<int*>(q.T.inds[f]);
+        gd.w[f] = const_cast<double*>(q.T.w[f]);
+    }
+    b->grids.push_back(gd);
+    b->inst.resize(1);
+    b->inst[0].split = q.split;
+    b->G = q.G;
+    b->maxm = q.T.m;
+    b->lds_rhs = rhs_lds_bytes(N, (N + q.split - 1) / q.split, 0);
+    b->ept = (N + 255) / 256;
+    std::vector<int2> smap{make_int2(0, -2), make_int2(0, -1)};
+    for (int k = 0; k < N; ++k) smap.push_back(make_int2(0, k));
+    std::vector<int4> rmap;
+    for (int j = 0; j < N; ++j)
+        for (int ch = 0; ch < q.split; ++ch) rmap.push_back(make_int4(0, j, ch, 0));
+    while (rmap.size() % 8) rmap.push_back(make_int4(-1, 0, 0, 0));
+    b->nb_stage = (int)smap.size();
+    b->nb_rhs = (int)rmap.size();
+    b->ctl.assign(1, Ctl{});
+    if (hipMalloc(&b->d_desc, sizeof(InstDesc)) != hipSuccess || hipMalloc(&b->d_ctl, sizeof(Ctl)) != hipSuccess ||
+        hipMalloc(&b->d_smap, smap.size() * sizeof(int2)) != hipSuccess ||
+        hipMalloc(&b->d_rmap, rmap.size() * sizeof(int4)) != hipSuccess ||
+        hipMalloc(&b->d_obs, (size_t)kLoopChunk * 4 * sizeof(double)) != hipSuccess) {
+        free_batch(b);
+        return set_error(DMFE_ERR_NOMEM, "dmfe_run: device loop allocation failed");
+    }
+    b->obs_cap = kLoopChunk;
+    if (hipMemcpyAsync(b->d_smap, smap.data(), smap.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->d_rmap, rmap.data(), rmap.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream) != hipSuccess) {
+        free_batch(b);
+        return set_error(DMFE_ERR_HIP, "dmfe_run: device loop map upload failed");
+    }
+    *out = b;
+    return DMFE_OK;
+}
+}  // namespace
+
+void device_loop_close(dmfe_batch* b) { free_batch(b); }
+
+int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_loop, double* corr, int64_t corr_cap,
+                    int64_t* steps) {
+    dmfe_batch* b = *slot;
+    if (b && (b->G != q.G || b->inst[0].split != q.split || b->stream != q.stream)) {
+        free_batch(b);  // launch shape changed: maps and captured graphs are stale
+        *slot = b = nullptr;
+    }
+    if (!b) {
+        int rc = device_loop_open(q, &b);
+        if (rc) return rc;
+        *slot = b;
+    }
+    Inst& x = b->inst[0];
+    x.gi = 0; x.prm = q.prm; x.M = q.M; x.H = q.H; x.W = q.W; x.cap = q.cap;
+    const bool rec = corr && corr_cap > 0;
+    if (rec != b->obs_on) b->obs_on = rec;  // build_graph re-captures when the recording flag changes
+    Ctl& c = b->ctl[0];
+    c = Ctl{};
+    c.status = kRunning;
+    c.cap = (int)q.cap;
+    c.dt_min = q.prm.delta_t_min;
+    c.delta_max = q.prm.delta_max;
+    c.specRad = 4 * std::sqrt(host_fl(q.prm, 1, 2));
+    std::memcpy(c.rmax, q.prm.rmax, sizeof c.rmax);
+    c.tmax = tmax;
+    c.max_loop = max_loop;
+    c.maint_every = q.prm.maint_every > 0 ? q.prm.maint_every : 100000;
+    c.T1 = (int)q.T1;
+    c.method = q.method;
+    c.fcur = q.fcur;
+    c.loop = q.loop;
+    c.dt = q.dt;
+    c.delta = q.delta;
+    c.delta_old = q.delta_old;
+    c.t_prop = q.t_last;
+    c.active = 0;
+    int rc = upload_desc(b);
+    if (!rc) rc = push_ctl(b);
+    const int64_t loop0 = q.loop;
+    std::vector<double> ring(rec ? (size_t)kLoopChunk * 4 : 0);
+    while (!rc) {
+        const int S = max_stages(b);
+        if (S == 0) break;
+        const int which = S == 7 ? 0 : 1;
+        const int64_t before = b->ctl[0].loop;
+        rc = build_graph(b, which, S, kLoopChunk);
+        if (rc) break;
+        hipError_t e = hipGraphLaunch(b->graph[which], b->stream);
+        if (e != hipSuccess) {
+            rc = set_error(DMFE_ERR_HIP, std::string("dmfe_run: graph launch: ") + hipGetErrorString(e));
+            break;
+        }
+        rc = pull_ctl(b);
+        if (rc) break;
+        if (rec) {
+            e = hipMemcpyAsync(ring.data(), b->d_obs, ring.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+            if (e != hipSuccess) {
+                rc = set_error(DMFE_ERR_HIP, std::string("dmfe_run: records: ") + hipGetErrorString(e));
+                break;
+            }
+            for (int64_t lp = before + 1; lp <= b->ctl[0].loop; ++lp) {
+                const int64_t k = lp - loop0 - 1;  // row of corr
+                if (k >= corr_cap) break;
+                const double* o = ring.data() + (size_t)((lp - 1) % kLoopChunk) * 4;
+                corr[3 * k + 0] = o[0];
+                corr[3 * k + 1] = o[1];
+                corr[3 * k + 2] = o[2];
+            }
+        }
+        bool changed = false;
+        rc = service(b, &changed);
+    }
+    // the caller adopts the (possibly replaced) arena and the control state
+    q.H = x.H;
+    q.cap = x.cap;
+    const Ctl& f = b->ctl[0];
+    q.T1 = f.T1; q.loop = f.loop; q.method = f.method; q.fcur = f.fcur;
+    q.dt = f.dt; q.delta = f.delta; q.delta_old = f.delta_old;
+    if (steps) *steps = f.loop - loop0;
+    return rc;
+}
+
+}  // namespace dmfe
 
 extern "C" int dmfe_batch_create(int n, const dmfe_grid* grids, const dmfe_params* params, int64_t rows_cap,
                                  dmfe_batch** out) {

@@ -14,8 +14,10 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -125,6 +127,7 @@ struct dmfe_ctx {
     long long P = 0;
     int split = 2;
     int G = 8;                // lanes per B2 entry of the RHS kernel
+    dmfe_batch* dloop = nullptr;  // dmfe_run's device loop (launch maps, captured graphs)
     bool dense_mfma = false;  // dense (B-spline) grid: A1/A2 contracted on MFMA, weights entry-major
     // forward-mode sensitivities (dmfe_tangent_init): tangent histories and work
     int tanK = 0;
@@ -561,6 +564,8 @@ int observables(dmfe_ctx* c, double t, double* E, double* qk0) {
 
 void free_ctx(dmfe_ctx* c) {
     if (!c) return;
+    if (c->dloop) device_loop_close(c->dloop);  // waits for the stream; frees only the loop's own buffers
+    c->dloop = nullptr;
     hipFree(c->theta); hipFree(c->integ); hipFree(c->phi2);
     for (int f = 0; f < 3; ++f) { hipFree(c->inds[f]); hipFree(c->w[f]); }
     free_history(c->H);
@@ -877,8 +882,57 @@ extern "C" int dmfe_loop_iteration(dmfe_ctx* c, double* t_out) {
     return loop_iteration(c, t_out);
 }
 
+// dmfe_run on the device-side step control: the whole main loop runs as replayed
+// hipGraphs of 16 steps with the dt rule on the device; the host only services arena
+// growth and the maintenance cadence between replays.  Same arithmetic as the host loop.
+static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap,
+                           int64_t* steps) {
+    if (!c->interp_valid && c->T1 > 1) {
+        int rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    DeviceLoop q{};
+    q.stream = c->stream;
+    q.T = c->tables();
+    q.W = c->W;
+    q.M = c->model;
+    q.prm = c->params;
+    q.split = c->split;
+    q.G = c->G;
+    q.H = c->H;
+    q.cap = c->cap;
+    q.T1 = c->T1;
+    q.loop = c->loop;
+    q.method = c->method;
+    q.fcur = c->fcur;
+    q.dt = c->dt;
+    q.delta = c->delta;
+    q.delta_old = c->delta_old;
+    q.t_last = c->t1[c->T1 - 1];
+    const int rc = device_loop_run(&c->dloop, q, tmax, max_loop, corr, corr_cap, steps);
+    c->H = q.H;
+    c->cap = q.cap;
+    c->T1 = q.T1;
+    c->loop = q.loop;
+    c->method = q.method;
+    c->fcur = q.fcur;
+    c->dt = q.dt;
+    c->delta = q.delta;
+    c->delta_old = q.delta_old;
+    c->interp_valid = true;
+    // host mirrors of t1grid and delta_t_ratio
+    c->t1.resize((size_t)c->T1);
+    c->rho.resize((size_t)c->T1);
+    HIP_TRY(hipMemcpyAsync(c->t1.data(), c->H.t1, (size_t)c->T1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->rho.data(), c->H.rho, (size_t)c->T1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
 extern "C" int dmfe_run(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps) {
     if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_run: no state");
+    // tangents, dense MFMA contexts and per-launch profiling keep the host-driven loop
+    if (!c->tanK && !c->dense_mfma && !c->profiling) return run_device_loop(c, tmax, max_loop, corr, corr_cap, steps);
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];
     const double eps = std::numeric_limits<double>::epsilon();
@@ -1021,6 +1075,42 @@ extern "C" int dmfe_gradients(dmfe_ctx* c, double tmax, int64_t max_loop, double
     grad[0] = dE[0]; grad[1] = dE[1]; grad[2] = dQ[0]; grad[3] = dQ[1];
     if (t_final) *t_final = t;
     if (steps) *steps = done;
+    return DMFE_OK;
+}
+
+// n (Gamma, lambda) points at once: one context per point, each on its own stream and
+// host thread (min(n, 16) workers), so the points' kernels overlap on the device.
+extern "C" int dmfe_gradients_many(int n, const dmfe_grid* grids, const dmfe_params* params, double tmax,
+                                   int64_t max_loop, int64_t rows_hint, double* grads, double* t_final,
+                                   int64_t* steps) {
+    if (n < 1 || !grids || !params || !grads) return set_error(DMFE_ERR_ARG, "dmfe_gradients_many: bad argument");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::vector<int> rc((size_t)n, DMFE_OK);
+    std::vector<std::string> msg((size_t)n);
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        (void)hipSetDevice(dev);
+        for (int i = next++; i < n; i = next++) {
+            dmfe_ctx* c = nullptr;
+            double tf = 0.0;
+            int64_t st = 0;
+            int r = dmfe_ctx_create(&grids[i], &params[i], rows_hint, &c);
+            if (r == DMFE_OK) r = dmfe_gradients(c, tmax, max_loop, grads + 4 * (size_t)i, &tf, &st);
+            if (r != DMFE_OK) msg[(size_t)i] = dmfe_last_error();
+            if (c) dmfe_ctx_destroy(c);
+            rc[(size_t)i] = r;
+            if (t_final) t_final[i] = tf;
+            if (steps) steps[i] = st;
+        }
+    };
+    const int nw = std::min(n, 16);
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nw; ++w) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    for (int i = 0; i < n; ++i)
+        if (rc[(size_t)i] != DMFE_OK)
+            return set_error(rc[(size_t)i], "dmfe_gradients_many: point " + std::to_string(i) + ": " + msg[(size_t)i]);
     return DMFE_OK;
 }
 

@@ -242,6 +242,30 @@ const Tableau& dp54();
 const Tableau& ssprk104();
 double host_fl(const dmfe_params& p, double q, int d);
 void batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk);
+
+// The main loop of one quench on the device-side step control (the batch machinery with
+// one instance), borrowing the caller's tables, work buffers and stream.  The history
+// arena is the caller's too, but it may be replaced (growth, sparsification): H and cap
+// come back updated, the old arena freed.  Control scalars are in/out.
+struct DeviceLoop {
+    hipStream_t stream;
+    Tables T;
+    Work W;
+    Model M;
+    dmfe_params prm;
+    int split, G;
+    History H;
+    int64_t cap;
+    int64_t T1, loop;
+    int method, fcur;
+    double dt, delta, delta_old;
+    double t_last;  // t1grid.back() on entry
+};
+// *slot caches the loop's launch maps and captured graphs between calls (NULL at first;
+// released with device_loop_close).
+int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_loop, double* corr, int64_t corr_cap,
+                    int64_t* steps);
+void device_loop_close(dmfe_batch* b);
 void launch_batch_begin(InstDesc* D, int n, hipStream_t s);
 void launch_batch_end(InstDesc* D, int n, hipStream_t s);
 void launch_batch_stage(InstDesc* D, const int2* map, int nblocks, int n, int ept, hipStream_t s);

@@ -238,6 +238,13 @@ int dmfe_tangent_state(dmfe_ctx* ctx, int k, double* dQK, double* dQR, double* d
 /* Quench + run (as dmfe_run) carrying both directions; grad = [dE/dGamma, dE/dlambda,
  * dQK0/dGamma, dQK0/dlambda] at the final proposed time *t_final. */
 int dmfe_gradients(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* grad, double* t_final, int64_t* steps);
+/* dmfe_gradients for n (grid, params) points at once (no counterpart in the reference,
+ * which runs one parameter point per process): each point gets its own context, stream
+ * and host thread (at most 16 at a time), so their kernels overlap on the device.
+ * grads [n][4] as in dmfe_gradients; t_final [n], steps [n] may be NULL.  The results
+ * are those of dmfe_gradients on a context of each point. */
+int dmfe_gradients_many(int n, const dmfe_grid* grids, const dmfe_params* params, double tmax, int64_t max_loop,
+                        int64_t rows_hint, double* grads, double* t_final, int64_t* steps);
 
 /* Dense (B-spline, m = N) stencil table `map` (0 A1, 1 A2, 2 B2) applied to B <= 16
  * right-hand sides as one FP64 MFMA contraction: Y [N*N][B] = W [N*N][N] . X [N][B]

@@ -150,3 +150,17 @@ def test_torch_autograd_binding(nominal):
     want = Jfd[0] + 3.0 * Jfd[1]
     got = np.array([G.grad.item(), lam.grad.item()])
     assert np.all(np.abs(got - want) <= 1e-6 * np.abs(want).max() + 1e-9), (got, want)
+
+
+def test_gradients_many_matches_single_points():
+    """Several (Gamma, lambda) points at once (one context, stream and host thread each)
+    give exactly the gradients of one context per point."""
+    pts = [(0.7, 0.5), (0.4, 0.2), (0.9, 0.8), (0.55, 0.35), (0.3, 0.6)]
+    prm = [params(G, lam) for G, lam in pts]
+    grads, tf, st = dm.gradients_many([grid()] * len(pts), prm, 1e300, STEPS)
+    for i, p in enumerate(prm):
+        c = dm.Context(grid(), p)
+        g1, t1, n1 = c.gradients(1e300, STEPS)
+        c.close()
+        assert np.array_equal(grads[i], g1), (i, grads[i], g1)
+        assert tf[i] == t1 and st[i] == n1
