@@ -75,6 +75,7 @@ struct Inst {
     Work W{};
     int split = 1;
     int64_t cap = 0;
+    TanArgs X{};  // tangents (device loop of a context only; X.K = 0 otherwise)
 };
 
 }  // namespace
@@ -90,6 +91,7 @@ struct dmfe_batch {
     int2* d_smap = nullptr;
     int4* d_rmap = nullptr;
     int nb_stage = 0, nb_rhs = 0, ept = 1, G = 8, maxm = 1, log_response = 0;
+    int tanK = 0;  // device loop of a context with tangents: tangent RHS and finish launches
     size_t lds_rhs = 0;
     double* d_obs = nullptr;
     int obs_cap = 0;
@@ -115,6 +117,7 @@ int upload_desc(dmfe_batch* b) {
         d[i].M = x.M;
         d[i].ctl = b->d_ctl + i;
         d[i].split = x.split;
+        d[i].X = x.X;
     }
     HIP_TRY(hipMemcpyAsync(b->d_desc, d.data(), d.size() * sizeof(InstDesc), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -166,7 +169,13 @@ int enqueue_step(dmfe_batch* b, int S) {
     launch_batch_begin(b->d_desc, n, b->stream);
     for (int k = 0; k < S; ++k) {
         launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);
-        launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->maxm, b->lds_rhs, b->stream);
+        if (b->tanK) {
+            // before the primal finish of stage k (next stage launch), as in the host loop
+            launch_batch_rhs_tan(b->d_desc, b->d_rmap, b->nb_rhs, k, b->G, b->tanK, b->lds_rhs, b->stream);
+            launch_batch_tan_finish(b->d_desc, 0, k, b->grids[0].N, b->tanK, b->stream);
+        } else {
+            launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->maxm, b->lds_rhs, b->stream);
+        }
     }
     launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, S, b->ept, b->stream);
     launch_batch_end(b->d_desc, n, b->stream);
@@ -187,7 +196,9 @@ int build_graph(dmfe_batch* b, int which, int S, int steps) {
     if (b->graph[which]) (void)hipGraphExecDestroy(b->graph[which]);
     b->graph[which] = nullptr;
     hipGraph_t g = nullptr;
-    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    // relaxed: other host threads (contexts driven concurrently) may allocate or copy
+    // while this stream captures; the captured region itself holds kernel launches only
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
     int rc = DMFE_OK;
     for (int s = 0; s < steps && rc == DMFE_OK; ++s) rc = enqueue_step(b, S);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -306,6 +317,8 @@ int service(dmfe_batch* b, bool* changed) {
     for (int i = 0; i < n_inst(b); ++i) {
         Ctl& c = b->ctl[(size_t)i];
         int rc = DMFE_OK;
+        if (b->inst[(size_t)i].X.K && (c.status == kNeedMaint || c.status == kNeedRows))
+            return set_error(DMFE_ERR_STATE, "device loop with tangents: arena or maintenance point reached");
         if (c.status == kNeedMaint) rc = maintenance(b, i);
         else if (c.status == kNeedRows) rc = grow_inst(b, i);
         else continue;
@@ -428,7 +441,8 @@ int device_loop_open(const DeviceLoop& q, dmfe_batch** out) {
     b->inst[0].split = q.split;
     b->G = q.G;
     b->maxm = q.T.m;
-    b->lds_rhs = rhs_lds_bytes(N, (N + q.split - 1) / q.split, 0);
+    b->tanK = q.X.K;
+    b->lds_rhs = rhs_lds_bytes(N, (N + q.split - 1) / q.split, q.X.K);
     b->ept = (N + 255) / 256;
     std::vector<int2> smap{make_int2(0, -2), make_int2(0, -1)};
     for (int k = 0; k < N; ++k) smap.push_back(make_int2(0, k));
@@ -462,7 +476,7 @@ void device_loop_close(dmfe_batch* b) { free_batch(b); }
 int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_loop, double* corr, int64_t corr_cap,
                     int64_t* steps) {
     dmfe_batch* b = *slot;
-    if (b && (b->G != q.G || b->inst[0].split != q.split || b->stream != q.stream)) {
+    if (b && (b->G != q.G || b->inst[0].split != q.split || b->stream != q.stream || b->tanK != q.X.K)) {
         free_batch(b);  // launch shape changed: maps and captured graphs are stale
         *slot = b = nullptr;
     }
@@ -472,7 +486,7 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
         *slot = b;
     }
     Inst& x = b->inst[0];
-    x.gi = 0; x.prm = q.prm; x.M = q.M; x.H = q.H; x.W = q.W; x.cap = q.cap;
+    x.gi = 0; x.prm = q.prm; x.M = q.M; x.H = q.H; x.W = q.W; x.cap = q.cap; x.X = q.X;
     const bool rec = corr && corr_cap > 0;
     if (rec != b->obs_on) b->obs_on = rec;  // build_graph re-captures when the recording flag changes
     Ctl& c = b->ctl[0];
@@ -493,7 +507,7 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
     c.dt = q.dt;
     c.delta = q.delta;
     c.delta_old = q.delta_old;
-    c.t_prop = q.t_last;
+    c.t_prop = q.t_prop;
     c.active = 0;
     int rc = upload_desc(b);
     if (!rc) rc = push_ctl(b);
@@ -504,12 +518,18 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
         if (S == 0) break;
         const int which = S == 7 ? 0 : 1;
         const int64_t before = b->ctl[0].loop;
-        rc = build_graph(b, which, S, kLoopChunk);
-        if (rc) break;
-        hipError_t e = hipGraphLaunch(b->graph[which], b->stream);
-        if (e != hipSuccess) {
-            rc = set_error(DMFE_ERR_HIP, std::string("dmfe_run: graph launch: ") + hipGetErrorString(e));
-            break;
+        hipError_t e = hipSuccess;
+        if (q.use_graph) {
+            rc = build_graph(b, which, S, kLoopChunk);
+            if (rc) break;
+            e = hipGraphLaunch(b->graph[which], b->stream);
+            if (e != hipSuccess) {
+                rc = set_error(DMFE_ERR_HIP, std::string("dmfe_run: graph launch: ") + hipGetErrorString(e));
+                break;
+            }
+        } else {
+            for (int s = 0; s < kLoopChunk && rc == DMFE_OK; ++s) rc = enqueue_step(b, S);
+            if (rc) break;
         }
         rc = pull_ctl(b);
         if (rc) break;
@@ -538,6 +558,7 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
     const Ctl& f = b->ctl[0];
     q.T1 = f.T1; q.loop = f.loop; q.method = f.method; q.fcur = f.fcur;
     q.dt = f.dt; q.delta = f.delta; q.delta_old = f.delta_old;
+    q.t_prop = f.t_prop;
     if (steps) *steps = f.loop - loop0;
     return rc;
 }

@@ -128,6 +128,7 @@ struct dmfe_ctx {
     int split = 2;
     int G = 8;                // lanes per B2 entry of the RHS kernel
     dmfe_batch* dloop = nullptr;  // dmfe_run's device loop (launch maps, captured graphs)
+    bool loop_graphs = true;      // dmfe_run replays captured graphs (dmfe_set_loop_graphs)
     bool dense_mfma = false;  // dense (B-spline) grid: A1/A2 contracted on MFMA, weights entry-major
     // forward-mode sensitivities (dmfe_tangent_init): tangent histories and work
     int tanK = 0;
@@ -885,41 +886,79 @@ extern "C" int dmfe_loop_iteration(dmfe_ctx* c, double* t_out) {
 // dmfe_run on the device-side step control: the whole main loop runs as replayed
 // hipGraphs of 16 steps with the dt rule on the device; the host only services arena
 // growth and the maintenance cadence between replays.  Same arithmetic as the host loop.
+// With tangents the run goes in segments that fit the arenas (grown here, tangents
+// included) and stops before a sparsification point, as loop_iteration does.
 static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap,
-                           int64_t* steps) {
+                           int64_t* steps, double* t_out = nullptr) {
     if (!c->interp_valid && c->T1 > 1) {
         int rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
         if (rc) return rc;
     }
-    DeviceLoop q{};
-    q.stream = c->stream;
-    q.T = c->tables();
-    q.W = c->W;
-    q.M = c->model;
-    q.prm = c->params;
-    q.split = c->split;
-    q.G = c->G;
-    q.H = c->H;
-    q.cap = c->cap;
-    q.T1 = c->T1;
-    q.loop = c->loop;
-    q.method = c->method;
-    q.fcur = c->fcur;
-    q.dt = c->dt;
-    q.delta = c->delta;
-    q.delta_old = c->delta_old;
-    q.t_last = c->t1[c->T1 - 1];
-    const int rc = device_loop_run(&c->dloop, q, tmax, max_loop, corr, corr_cap, steps);
-    c->H = q.H;
-    c->cap = q.cap;
-    c->T1 = q.T1;
-    c->loop = q.loop;
-    c->method = q.method;
-    c->fcur = q.fcur;
-    c->dt = q.dt;
-    c->delta = q.delta;
-    c->delta_old = q.delta_old;
-    c->interp_valid = true;
+    const int64_t loop0 = c->loop;
+    const int64_t every = c->params.maint_every > 0 ? c->params.maint_every : 100000;
+    double t_prop = c->t1[c->T1 - 1];
+    int rc = DMFE_OK;
+    for (;;) {
+        int64_t lim = max_loop;
+        bool at_maint = false;
+        if (c->tanK) {
+            rc = ensure_rows(c, c->T1 + std::max<int64_t>(256, c->T1 / 2));
+            if (rc) break;
+            lim = std::min<int64_t>(lim, c->loop + (c->cap - c->T1 - 1));
+            const int64_t next_maint = (c->loop / every + 1) * every;
+            if (lim >= next_maint) {
+                lim = next_maint - 1;
+                at_maint = true;
+            }
+        }
+        DeviceLoop q{};
+        q.stream = c->stream;
+        q.T = c->tables();
+        q.W = c->W;
+        q.M = c->model;
+        q.prm = c->params;
+        q.split = c->split;
+        q.G = c->G;
+        q.use_graph = c->loop_graphs ? 1 : 0;
+        if (c->tanK) q.X = c->tan_args();
+        q.H = c->H;
+        q.cap = c->cap;
+        q.T1 = c->T1;
+        q.loop = c->loop;
+        q.method = c->method;
+        q.fcur = c->fcur;
+        q.dt = c->dt;
+        q.delta = c->delta;
+        q.delta_old = c->delta_old;
+        q.t_prop = t_prop;
+        const int64_t done = c->loop - loop0;
+        double* cseg = (corr && done < corr_cap) ? corr + 3 * done : nullptr;
+        rc = device_loop_run(&c->dloop, q, tmax, lim, cseg, cseg ? corr_cap - done : 0, nullptr);
+        c->H = q.H;
+        c->cap = q.cap;
+        c->T1 = q.T1;
+        c->loop = q.loop;
+        c->method = q.method;
+        c->fcur = q.fcur;
+        c->dt = q.dt;
+        c->delta = q.delta;
+        c->delta_old = q.delta_old;
+        c->interp_valid = true;
+        t_prop = q.t_prop;
+        if (rc || !c->tanK || c->loop < lim || lim >= max_loop) break;
+        // a segment ended at its own limit: would the main loop go on?
+        const double eps = std::numeric_limits<double>::epsilon();
+        const bool more = t_prop < tmax &&
+                          c->dt >= c->params.delta_t_min - eps * std::max(std::fabs(c->dt), std::fabs(c->params.delta_t_min));
+        if (!more) break;
+        if (at_maint && c->loop == lim) {
+            rc = set_error(DMFE_ERR_STATE, "sensitivities: the run would cross a sparsification point (loop " +
+                                               std::to_string(c->loop + 1) + ")");
+            break;
+        }
+    }
+    if (steps) *steps = c->loop - loop0;
+    if (t_out) *t_out = t_prop;
     // host mirrors of t1grid and delta_t_ratio
     c->t1.resize((size_t)c->T1);
     c->rho.resize((size_t)c->T1);
@@ -929,10 +968,16 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
     return rc;
 }
 
+extern "C" int dmfe_set_loop_graphs(dmfe_ctx* c, int on) {
+    if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_loop_graphs: null context");
+    c->loop_graphs = on != 0;
+    return DMFE_OK;
+}
+
 extern "C" int dmfe_run(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps) {
     if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_run: no state");
-    // tangents, dense MFMA contexts and per-launch profiling keep the host-driven loop
-    if (!c->tanK && !c->dense_mfma && !c->profiling) return run_device_loop(c, tmax, max_loop, corr, corr_cap, steps);
+    // dense MFMA contexts and per-launch profiling keep the host-driven loop
+    if (!c->dense_mfma && !c->profiling) return run_device_loop(c, tmax, max_loop, corr, corr_cap, steps);
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];
     const double eps = std::numeric_limits<double>::epsilon();
@@ -1062,12 +1107,17 @@ extern "C" int dmfe_gradients(dmfe_ctx* c, double tmax, int64_t max_loop, double
     if (rc) return rc;
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];
-    const double eps = std::numeric_limits<double>::epsilon();
-    while (t < tmax && c->loop < max_loop &&
-           c->dt >= c->params.delta_t_min - eps * std::max(std::fabs(c->dt), std::fabs(c->params.delta_t_min))) {
-        rc = loop_iteration(c, &t);
+    if (!c->dense_mfma && !c->profiling) {
+        rc = run_device_loop(c, tmax, max_loop, nullptr, 0, &done, &t);
         if (rc) return rc;
-        ++done;
+    } else {
+        const double eps = std::numeric_limits<double>::epsilon();
+        while (t < tmax && c->loop < max_loop &&
+               c->dt >= c->params.delta_t_min - eps * std::max(std::fabs(c->dt), std::fabs(c->params.delta_t_min))) {
+            rc = loop_iteration(c, &t);
+            if (rc) return rc;
+            ++done;
+        }
     }
     double dE[2], dQ[2];
     rc = dmfe_tangent_observables(c, t, dE, dQ);
@@ -1096,6 +1146,8 @@ extern "C" int dmfe_gradients_many(int n, const dmfe_grid* grids, const dmfe_par
             double tf = 0.0;
             int64_t st = 0;
             int r = dmfe_ctx_create(&grids[i], &params[i], rows_hint, &c);
+            // no stream capture while other workers copy and allocate
+            if (r == DMFE_OK) r = dmfe_set_loop_graphs(c, 0);
             if (r == DMFE_OK) r = dmfe_gradients(c, tmax, max_loop, grads + 4 * (size_t)i, &tf, &st);
             if (r != DMFE_OK) msg[(size_t)i] = dmfe_last_error();
             if (c) dmfe_ctx_destroy(c);

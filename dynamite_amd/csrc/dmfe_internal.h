@@ -229,6 +229,7 @@ struct InstDesc {
     Ctl* ctl;
     int split;
     int pad;
+    TanArgs X;  // forward-mode tangents carried by the instance (X.K = 0: none)
 };
 
 // Butcher tableaux of the two integrators and f^(d)(q) on the host (dmfe_ctx.cpp)
@@ -254,12 +255,14 @@ struct DeviceLoop {
     Model M;
     dmfe_params prm;
     int split, G;
+    TanArgs X;      // X.K tangent directions carried along (their arenas must hold the run)
+    int use_graph;  // replay captured 16-step graphs (else the same launches directly)
     History H;
     int64_t cap;
     int64_t T1, loop;
     int method, fcur;
     double dt, delta, delta_old;
-    double t_last;  // t1grid.back() on entry
+    double t_prop;  // in: the loop variable t (t1grid.back() at a fresh call); out: last proposed t
 };
 // *slot caches the loop's launch maps and captured graphs between calls (NULL at first;
 // released with device_loop_close).
@@ -272,5 +275,8 @@ void launch_batch_stage(InstDesc* D, const int2* map, int nblocks, int n, int ep
 void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_response, int G, int maxm, size_t lds,
                       hipStream_t s);
 void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t s);
+// the RHS with K tangent directions, and the tangent finish of stage n, of instance `inst`
+void launch_batch_rhs_tan(InstDesc* D, const int4* map, int nblocks, int n, int G, int K, size_t lds, hipStream_t s);
+void launch_batch_tan_finish(InstDesc* D, int inst, int n, int N, int K, hipStream_t s);
 
 }  // namespace dmfe

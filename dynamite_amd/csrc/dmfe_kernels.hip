@@ -1445,6 +1445,27 @@ __global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* 
     rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red);
 }
 
+// the same RHS launch with the tangent quadratures of the instance's K directions
+template <int G, int K>
+__global__ __launch_bounds__(kRowThreads) void batch_rhs_tan_kernel(const InstDesc* __restrict__ D,
+                                                                    const int4* __restrict__ map, int n) {
+    extern __shared__ double2 dyn[];
+    __shared__ double red[kSums * (kRowThreads / kWave)];
+    const int4 mp = map[xcd_remap(blockIdx.x, gridDim.x)];
+    if (mp.x < 0) return;
+    const InstDesc& d = D[mp.x];
+    const Ctl& c = *d.ctl;
+    if (!c.active || n >= c.S) return;
+    RhsArgs a{};
+    a.T1 = (n == 0) ? c.T1s : c.T1s + 1;
+    a.tl = c.gt[n];
+    a.split = d.split;
+    a.log_response = 0;
+    a.fields = nullptr;
+    a.fbuf = (c.fc0 + searches_before(c, n + 1)) & 1;
+    rhs_rows_body<false, G, K>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red, &d.X);
+}
+
 // Observables of the instances that just stepped: (t_prop, E(t_prop), QK(t,0)) per row
 // (simulation_runner.cpp:308-337), appended at row (loop - 1) % cap of obs[inst].
 __global__ __launch_bounds__(kStageThreads) void batch_obs_kernel(const InstDesc* __restrict__ D, double* obs, int cap) {
@@ -1555,9 +1576,9 @@ __global__ void tan_init_kernel(Model M, TanArgs X, int N) {
 // runs before the primal finish of the same stage (which overwrites the RHS's last row
 // at stages >= 1) and recomputes the primal quantities it needs.
 template <int EPT, int K>
-__global__ __launch_bounds__(kStageThreads) void tan_finish_kernel(Tables T, History H, Work W, Model M, FinishArgs a,
-                                                                   TanArgs X) {
-    __shared__ double red[6 * K * (kStageThreads / kWave)];
+__device__ __forceinline__ void tan_finish_body(const Tables& T, const History& H, const Work& W, const Model& M,
+                                                const FinishArgs& a, const TanArgs& X,
+                                                double* red /* 6 K kStageThreads / 64 */) {
     const int N = T.N, n = a.stage;
     const double t = a.t_rhs;
     const double2* last = H.V + (size_t)a.lr * N;
@@ -1723,6 +1744,25 @@ __global__ __launch_bounds__(kStageThreads) void tan_finish_kernel(Tables T, His
             X.dr[h][a.wrow] = a.tdiff * tdr[h];
         }
     }
+}
+
+template <int EPT, int K>
+__global__ __launch_bounds__(kStageThreads) void tan_finish_kernel(Tables T, History H, Work W, Model M, FinishArgs a,
+                                                                   TanArgs X) {
+    __shared__ double red[6 * K * (kStageThreads / kWave)];
+    tan_finish_body<EPT, K>(T, H, W, M, a, X, red);
+}
+
+// device-side step control with tangents: the tangent finish of stage n of instance
+// `inst` from the stage arguments its begin kernel wrote (the RHS buffer of stage n)
+template <int EPT, int K>
+__global__ __launch_bounds__(kStageThreads) void batch_tan_finish_kernel(const InstDesc* __restrict__ D, int inst,
+                                                                         int n) {
+    __shared__ double red[6 * K * (kStageThreads / kWave)];
+    const InstDesc& d = D[inst];
+    const Ctl& c = *d.ctl;
+    if (!c.active || n >= c.S) return;
+    tan_finish_body<EPT, K>(d.T, d.H, d.W, d.M, c.fin[n], d.X, red);
 }
 
 // Tangent of the energy at time t on history row `row` (simulation_runner.cpp:308-318)
@@ -1928,6 +1968,36 @@ void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_
     hipLaunchKernelGGL((batch_rhs_kernel<LOG, GG, NT>), dim3(nblocks), dim3(kRowThreads), lds, s, D, map, n)
     DMFE_SHAPES(log_response, G, b2_taps_per_pass(G, maxm), DMFE_BRHS)
 #undef DMFE_BRHS
+}
+
+void launch_batch_rhs_tan(InstDesc* D, const int4* map, int nblocks, int n, int G, int K, size_t lds, hipStream_t s) {
+#define DMFE_BRT(GG, KK) \
+    hipLaunchKernelGGL((batch_rhs_tan_kernel<GG, KK>), dim3(nblocks), dim3(kRowThreads), lds, s, D, map, n)
+#define DMFE_BRTG(KK)                       \
+    switch (G) {                            \
+        case 4: DMFE_BRT(4, KK); break;     \
+        case 8: DMFE_BRT(8, KK); break;     \
+        case 16: DMFE_BRT(16, KK); break;   \
+        default: DMFE_BRT(64, KK); break;   \
+    }
+    if (K == 1) DMFE_BRTG(1)
+    else DMFE_BRTG(2)
+#undef DMFE_BRTG
+#undef DMFE_BRT
+}
+
+void launch_batch_tan_finish(InstDesc* D, int inst, int n, int N, int K, hipStream_t s) {
+    const int ept = (N + kStageThreads - 1) / kStageThreads;
+#define DMFE_BTF(E, KK) hipLaunchKernelGGL((batch_tan_finish_kernel<E, KK>), dim3(1), dim3(kStageThreads), 0, s, D, inst, n)
+#define DMFE_BTFE(KK)                       \
+    if (ept <= 1) DMFE_BTF(1, KK);          \
+    else if (ept <= 2) DMFE_BTF(2, KK);     \
+    else if (ept <= 4) DMFE_BTF(4, KK);     \
+    else DMFE_BTF(8, KK);
+    if (K == 1) { DMFE_BTFE(1) }
+    else { DMFE_BTFE(2) }
+#undef DMFE_BTFE
+#undef DMFE_BTF
 }
 
 void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t s) {

@@ -202,8 +202,17 @@ int dmfe_loop_iteration(dmfe_ctx* ctx, double* t_out);
 /* Runs loop iterations while t < tmax, loop < max_loop and delta_t >= delta_t_min
  * (simulation_runner.cpp:125).  If corr is non-NULL it receives up to corr_cap
  * rows (t, E, QK(t,0)) - the correlation.txt columns (:327-341).  *steps = number
- * of iterations run. */
+ * of iterations run.  The loop runs with its step control on the device (stage
+ * arguments, error norm and dt rule evaluated by kernels, 16 steps per host round
+ * trip); results are bit-identical to repeated dmfe_loop_iteration calls.  Contexts
+ * on dense grids and contexts with per-launch profiling on use the host loop. */
 int dmfe_run(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps);
+
+/* dmfe_run replays captured hipGraphs of its 16-step launch sequence (on, default) or
+ * issues the same launches directly (off).  No counterpart in the reference.  Turn it
+ * off for contexts that host threads drive concurrently: a stream capture in one
+ * thread makes pageable copies issued meanwhile by other threads fail. */
+int dmfe_set_loop_graphs(dmfe_ctx* ctx, int on);
 
 /* Energy at time t and QK(t,0) on the last history row (simulation_runner.cpp:308-337). */
 int dmfe_observables(dmfe_ctx* ctx, double t, double* energy, double* qk0);

@@ -99,3 +99,66 @@ def test_run_in_pieces_and_from_set_state():
     same_state(a, b)
     a.close()
     b.close()
+
+
+def test_run_with_tangents_matches_host_loop():
+    """Tangents on the device loop (tangent RHS and finish launches in the replayed
+    step), in segments across arena growth: primal and tangent histories and the tangent
+    observables equal the host-driven loop's bit for bit."""
+    g = grid(64)
+    dev = dm.Context(g, params(Gamma=0.7), rows_hint=32)
+    host = dm.Context(g, params(Gamma=0.7), rows_hint=32)
+    for c in (dev, host):
+        c.init_quench()
+        c.tangent_init([1.0, 0.3], [0.0, -2.0])
+    n, corr = dev.run(float("inf"), 300, corr_cap=300)
+    ref = host_run(host, n)
+    assert n == 300 and np.array_equal(corr, ref)
+    same_state(dev, host)
+    for k in range(2):
+        a, b = dev.tangent_state(k), host.tangent_state(k)
+        for key in ("QK", "QR", "rvec"):
+            assert np.array_equal(a[key], b[key]), (k, key)
+    t = corr[-1, 0]
+    for x, y in zip(dev.tangent_observables(t), host.tangent_observables(t)):
+        assert np.array_equal(x, y)
+    dev.close()
+    host.close()
+
+
+def test_run_with_tangents_stops_before_sparsification():
+    g = grid(64)
+    out = []
+    for use_run in (True, False):
+        c = dm.Context(g, params(maint=50, Gamma=0.7), rows_hint=64)
+        c.init_quench()
+        k = c.get_control()
+        c.set_control(4e-5, k["delta"], k["delta_old"], k["loop"], k["method"])
+        c.tangent_init([1.0], [0.0])
+        with pytest.raises(dm.DmfeError, match="sparsification point"):
+            if use_run:
+                c.run(float("inf"), 200)
+            else:
+                for _ in range(200):
+                    c.loop_iteration()
+        assert c.get_control()["loop"] == 49
+        out.append((c.get_state(), c.tangent_state(0)))
+        c.close()
+    for k in ("t1grid", "QK", "QR"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert all(np.array_equal(out[0][1][key], out[1][1][key]) for key in ("QK", "QR", "rvec"))
+
+
+def test_run_with_direct_launches_matches_graph_replays():
+    g = grid(64)
+    out = []
+    for graphs in (True, False):
+        c = dm.Context(g, params(), rows_hint=64)
+        dm.set_loop_graphs(c, graphs)
+        c.init_quench()
+        n, corr = c.run(float("inf"), 150, corr_cap=150)
+        out.append((c.get_state(), corr))
+        c.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    for k in ("t1grid", "QK", "QR", "dQK", "dQR", "rvec"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
