@@ -475,10 +475,10 @@ def main():
     if not a.no_dense:
         out["dense_bspline"] = dense_microbench()
         log(f"[bench] dense: {out['dense_bspline']}")
+    if not a.no_sensitivity:  # before the batch, whose autotuning installs L=256 shapes
+        out["sensitivities"] = sensitivity_run()
     if not a.no_batch:
         out["batch"] = batch_run(a.batch_prep, a.batch_steps, a.looped_steps)
-    if not a.no_sensitivity:
-        out["sensitivities"] = sensitivity_run()
     if not a.no_cpu:
         log(f"[bench] CPU baseline (budget {a.cpu_budget:.0f}s)")
         out["cpu_baseline"] = cpu_baseline(g, state, control, a.cpu_budget)

@@ -101,11 +101,14 @@ struct dmfe_batch {
     bool graph_obs[2] = {false, false};
     int64_t steps_run = 0;
     bool borrowed = false;  // device_loop_run: tables, instance buffers and stream belong to a context
+    InstDesc desc0{};       // device loop: instance 0's descriptor, passed by value to its kernels
 };
 
 namespace {
 
 int n_inst(const dmfe_batch* b) { return (int)b->inst.size(); }
+
+void drop_graphs(dmfe_batch* b);
 
 int upload_desc(dmfe_batch* b) {
     std::vector<InstDesc> d(b->inst.size());
@@ -121,6 +124,10 @@ int upload_desc(dmfe_batch* b) {
     }
     HIP_TRY(hipMemcpyAsync(b->d_desc, d.data(), d.size() * sizeof(InstDesc), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->borrowed && (std::memcmp(&b->desc0, &d[0], sizeof(InstDesc)) != 0)) {
+        b->desc0 = d[0];
+        drop_graphs(b);  // the device loop's captured launches carry the descriptor by value
+    }
     return DMFE_OK;
 }
 
@@ -166,6 +173,20 @@ int interpolate_inst(dmfe_batch* b, int i) {
 // One batch step: begin | (stage n, rhs n) for n < S | stage S | end [| observables]
 int enqueue_step(dmfe_batch* b, int S) {
     const int n = n_inst(b);
+    if (b->borrowed) {  // one context: descriptor by value, no launch maps
+        const InstDesc& d = b->desc0;
+        launch_loop_begin(d, b->stream);
+        for (int k = 0; k < S; ++k) {
+            launch_loop_stage(d, k, b->stream);
+            launch_loop_rhs(d, k, b->log_response, b->G, b->lds_rhs, b->stream);
+            if (d.X.K) launch_loop_tan_finish(d, k, b->stream);
+        }
+        launch_loop_stage(d, S, b->stream);
+        launch_loop_end(d, b->stream);
+        if (b->obs_on) launch_batch_obs(b->d_desc, n, b->d_obs, b->obs_cap, b->stream);
+        HIP_TRY(hipGetLastError());
+        return DMFE_OK;
+    }
     launch_batch_begin(b->d_desc, n, b->stream);
     for (int k = 0; k < S; ++k) {
         launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);

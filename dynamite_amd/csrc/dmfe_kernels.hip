@@ -1323,11 +1323,9 @@ __device__ inline void make_finish_dev(const Ctl& c, const InstDesc& d, int m, F
 // One block per instance: lane 0 runs the scalar control, then lanes 0..S build the
 // search arguments of stages 0..S and lanes 16..16+S-1 the finish arguments.
 constexpr int kBeginThreads = 32;
-__global__ __launch_bounds__(kBeginThreads) void batch_begin_kernel(InstDesc* D, int n) {
-    const int i = blockIdx.x;
-    if (i >= n) return;
-    Ctl& c = *D[i].ctl;
-    __shared__ int go;
+__device__ __forceinline__ void begin_body(const InstDesc& d, int* go_lds) {
+    Ctl& c = *d.ctl;
+    int& go = *go_lds;
     if (threadIdx.x == 0) {
         go = 0;
         c.active = 0;
@@ -1339,7 +1337,7 @@ __global__ __launch_bounds__(kBeginThreads) void batch_begin_kernel(InstDesc* D,
         } else if (c.T1 + 1 > c.cap) {
             c.status = kNeedRows;
         } else {
-            const double* t1 = D[i].H.t1;
+            const double* t1 = d.H.t1;
             c.T1s = c.T1;
             c.gt[0] = t1[c.T1 - 1];
             c.tprev = c.T1 >= 2 ? t1[c.T1 - 2] : 0.0;
@@ -1367,18 +1365,27 @@ __global__ __launch_bounds__(kBeginThreads) void batch_begin_kernel(InstDesc* D,
     __syncthreads();
     if (!go) return;
     const int k = threadIdx.x;
-    if (k <= c.S) make_search_dev(c, D[i], k, c.srch[k]);
-    else if (k >= 16 && k - 16 < c.S) make_finish_dev(c, D[i], k - 16, c.fin[k - 16]);
+    if (k <= c.S) make_search_dev(c, d, k, c.srch[k]);
+    else if (k >= 16 && k - 16 < c.S) make_finish_dev(c, d, k - 16, c.fin[k - 16]);
     if (k == 0) c.active = 1;
 }
 
+__global__ __launch_bounds__(kBeginThreads) void batch_begin_kernel(InstDesc* D, int n) {
+    __shared__ int go;
+    if ((int)blockIdx.x < n) begin_body(D[blockIdx.x], &go);
+}
+
+// device loop of one context: the instance comes by value (no descriptor loads)
+__global__ __launch_bounds__(kBeginThreads) void loop_begin_kernel(InstDesc d) {
+    __shared__ int go;
+    begin_body(d, &go);
+}
+
 // end: delta, loop, maintenance cadence and the adaptive dt rule (simulation_runner.cpp:144-278)
-__global__ void batch_end_kernel(InstDesc* D, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Ctl& c = *D[i].ctl;
+__device__ __forceinline__ void end_body(const InstDesc& d) {
+    Ctl& c = *d.ctl;
     if (!c.active) return;  // stays set until the next begin: the observables launch reads it
-    c.delta = D[i].W.scal[1];
+    c.delta = d.W.scal[1];
     c.loop += 1;
     c.T1 = c.T1s + 1;
     c.fcur = (c.fc0 + searches_before(c, c.S) + 1) & 1;
@@ -1387,7 +1394,7 @@ __global__ void batch_end_kernel(InstDesc* D, int n) {
         return;
     }
     const double dmax = c.delta_max;
-    const double rho_last = D[i].H.rho[c.T1 - 1];
+    const double rho_last = d.H.rho[c.T1 - 1];
     if (c.delta < dmax && c.loop > 5 && (c.delta < 1.1 * c.delta_old || c.delta_old < dmax / 1000) &&
         c.rmax[c.method - 1] / c.specRad > c.dt && rho_last == 1.0) {
         c.dt *= 1.01;
@@ -1399,10 +1406,33 @@ __global__ void batch_end_kernel(InstDesc* D, int n) {
         c.dt *= 0.5;
     }
 }
+
+__global__ void batch_end_kernel(InstDesc* D, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) end_body(D[i]);
+}
+
+__global__ void loop_end_kernel(InstDesc d) {
+    if (threadIdx.x == 0) end_body(d);
+}
 #pragma clang fp contract(on)
 
 // Stage launch n of a batch step: map[b] = (instance, role); role -2 finish of stage
 // n-1, -1 theta-row search, k >= 0 phi2 row k of the interpolation of stage n.
+template <int EPT>
+__device__ __forceinline__ void stage_role_body(const InstDesc& d, int role, int n, double* lds, double* red) {
+    const Ctl& c = *d.ctl;
+    if (!c.active || n > c.S) return;
+    if (role == -2) {
+        if (n >= 1) finish_block<EPT>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
+        return;
+    }
+    const SearchArgs& a = c.srch[n];
+    if (a.mode < 0) return;
+    if (role == -1) search_theta_row<EPT>(d.T, d.W, a, lds, red);
+    else if (!(a.mode == kSearchSame && !a.saveB2)) search_phi2_row<EPT>(d.T, d.W, a, role, lds);
+}
+
 template <int EPT>
 __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(const InstDesc* __restrict__ D,
                                                                     const int2* __restrict__ map, int n) {
@@ -1410,21 +1440,30 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(const InstDe
     __shared__ double red[8 * (kStageThreads / kWave)];
     const int2 mp = map[blockIdx.x];
     if (mp.x < 0) return;
-    const InstDesc& d = D[mp.x];
-    const Ctl& c = *d.ctl;
-    if (!c.active || n > c.S) return;
-    if (mp.y == -2) {
-        if (n >= 1) finish_block<EPT>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
-        return;
-    }
-    const SearchArgs& a = c.srch[n];
-    if (a.mode < 0) return;
-    if (mp.y == -1) search_theta_row<EPT>(d.T, d.W, a, lds, red);
-    else if (!(a.mode == kSearchSame && !a.saveB2)) search_phi2_row<EPT>(d.T, d.W, a, mp.y, lds);
+    stage_role_body<EPT>(D[mp.x], mp.y, n, lds, red);
+}
+
+// device loop of one context: block 0 finish, block 1 theta row, block 2 + k phi2 row k
+template <int EPT>
+__global__ __launch_bounds__(kStageThreads) void loop_stage_kernel(InstDesc d, int n) {
+    __shared__ double lds[EPT * kStageThreads];
+    __shared__ double red[8 * (kStageThreads / kWave)];
+    stage_role_body<EPT>(d, (int)blockIdx.x - 2, n, lds, red);
 }
 
 // RHS launch n: map[b] = (instance, row j, chunk, -); blocks of one (j, chunk) for all
 // instances sharing a grid are adjacent, so their stencil weights are fetched once per XCD.
+__device__ __forceinline__ RhsArgs step_rhs_args(const Ctl& c, const InstDesc& d, int n, int logr) {
+    RhsArgs a{};
+    a.T1 = (n == 0) ? c.T1s : c.T1s + 1;
+    a.tl = c.gt[n];
+    a.split = d.split;
+    a.log_response = logr;
+    a.fields = nullptr;
+    a.fbuf = (c.fc0 + searches_before(c, n + 1)) & 1;
+    return a;
+}
+
 template <bool LOGR, int G, int NT>
 __global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* __restrict__ D,
                                                                 const int4* __restrict__ map, int n) {
@@ -1435,14 +1474,20 @@ __global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* 
     const InstDesc& d = D[mp.x];
     const Ctl& c = *d.ctl;
     if (!c.active || n >= c.S) return;
-    RhsArgs a{};
-    a.T1 = (n == 0) ? c.T1s : c.T1s + 1;
-    a.tl = c.gt[n];
-    a.split = d.split;
-    a.log_response = LOGR;
-    a.fields = nullptr;
-    a.fbuf = (c.fc0 + searches_before(c, n + 1)) & 1;
+    const RhsArgs a = step_rhs_args(c, d, n, LOGR);
     rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red);
+}
+
+// device loop of one context: the instance by value, block -> (row, chunk) as in rhs_rows_kernel
+template <bool LOGR, int G, int NT>
+__global__ __launch_bounds__(kRowThreads) void loop_rhs_kernel(InstDesc d, int n) {
+    extern __shared__ double2 dyn[];
+    __shared__ double red[kSums * (kRowThreads / kWave)];
+    const Ctl& c = *d.ctl;
+    if (!c.active || n >= c.S) return;
+    const RhsArgs a = step_rhs_args(c, d, n, LOGR);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red);
 }
 
 // the same RHS launch with the tangent quadratures of the instance's K directions
@@ -1456,14 +1501,19 @@ __global__ __launch_bounds__(kRowThreads) void batch_rhs_tan_kernel(const InstDe
     const InstDesc& d = D[mp.x];
     const Ctl& c = *d.ctl;
     if (!c.active || n >= c.S) return;
-    RhsArgs a{};
-    a.T1 = (n == 0) ? c.T1s : c.T1s + 1;
-    a.tl = c.gt[n];
-    a.split = d.split;
-    a.log_response = 0;
-    a.fields = nullptr;
-    a.fbuf = (c.fc0 + searches_before(c, n + 1)) & 1;
+    const RhsArgs a = step_rhs_args(c, d, n, 0);
     rhs_rows_body<false, G, K>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red, &d.X);
+}
+
+template <int G, int K>
+__global__ __launch_bounds__(kRowThreads) void loop_rhs_tan_kernel(InstDesc d, int n) {
+    extern __shared__ double2 dyn[];
+    __shared__ double red[kSums * (kRowThreads / kWave)];
+    const Ctl& c = *d.ctl;
+    if (!c.active || n >= c.S) return;
+    const RhsArgs a = step_rhs_args(c, d, n, 0);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    rhs_rows_body<false, G, K>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red, &d.X);
 }
 
 // Observables of the instances that just stepped: (t_prop, E(t_prop), QK(t,0)) per row
@@ -1765,6 +1815,14 @@ __global__ __launch_bounds__(kStageThreads) void batch_tan_finish_kernel(const I
     tan_finish_body<EPT, K>(d.T, d.H, d.W, d.M, c.fin[n], d.X, red);
 }
 
+template <int EPT, int K>
+__global__ __launch_bounds__(kStageThreads) void loop_tan_finish_kernel(InstDesc d, int n) {
+    __shared__ double red[6 * K * (kStageThreads / kWave)];
+    const Ctl& c = *d.ctl;
+    if (!c.active || n >= c.S) return;
+    tan_finish_body<EPT, K>(d.T, d.H, d.W, d.M, c.fin[n], d.X, red);
+}
+
 // Tangent of the energy at time t on history row `row` (simulation_runner.cpp:308-318)
 template <int K>
 __global__ __launch_bounds__(kStageThreads) void tan_energy_kernel(Tables T, History H, Model M, TanArgs X, int row,
@@ -1998,6 +2056,60 @@ void launch_batch_tan_finish(InstDesc* D, int inst, int n, int N, int K, hipStre
     else { DMFE_BTFE(2) }
 #undef DMFE_BTFE
 #undef DMFE_BTF
+}
+
+// ---- device loop of one context (instance by value) ----
+void launch_loop_begin(const InstDesc& d, hipStream_t s) {
+    hipLaunchKernelGGL(loop_begin_kernel, dim3(1), dim3(kBeginThreads), 0, s, d);
+}
+
+void launch_loop_end(const InstDesc& d, hipStream_t s) {
+    hipLaunchKernelGGL(loop_end_kernel, dim3(1), dim3(64), 0, s, d);
+}
+
+void launch_loop_stage(const InstDesc& d, int n, hipStream_t s) {
+    const int N = d.T.N, ept = (N + kStageThreads - 1) / kStageThreads;
+    const dim3 grid((unsigned)(N + 2));
+    if (ept <= 1) hipLaunchKernelGGL(loop_stage_kernel<1>, grid, dim3(kStageThreads), 0, s, d, n);
+    else if (ept <= 2) hipLaunchKernelGGL(loop_stage_kernel<2>, grid, dim3(kStageThreads), 0, s, d, n);
+    else if (ept <= 4) hipLaunchKernelGGL(loop_stage_kernel<4>, grid, dim3(kStageThreads), 0, s, d, n);
+    else hipLaunchKernelGGL(loop_stage_kernel<8>, grid, dim3(kStageThreads), 0, s, d, n);
+}
+
+void launch_loop_rhs(const InstDesc& d, int n, int log_response, int G, size_t lds, hipStream_t s) {
+    const dim3 grid((unsigned)(d.T.N * d.split));
+    if (d.X.K) {
+#define DMFE_LRT(GG, KK) hipLaunchKernelGGL((loop_rhs_tan_kernel<GG, KK>), grid, dim3(kRowThreads), lds, s, d, n)
+#define DMFE_LRTG(KK)                       \
+    switch (G) {                            \
+        case 4: DMFE_LRT(4, KK); break;     \
+        case 8: DMFE_LRT(8, KK); break;     \
+        case 16: DMFE_LRT(16, KK); break;   \
+        default: DMFE_LRT(64, KK); break;   \
+    }
+        if (d.X.K == 1) DMFE_LRTG(1)
+        else DMFE_LRTG(2)
+#undef DMFE_LRTG
+#undef DMFE_LRT
+        return;
+    }
+#define DMFE_LR(LOG, GG, NT) hipLaunchKernelGGL((loop_rhs_kernel<LOG, GG, NT>), grid, dim3(kRowThreads), lds, s, d, n)
+    DMFE_SHAPES(log_response, G, b2_taps_per_pass(G, d.T.m), DMFE_LR)
+#undef DMFE_LR
+}
+
+void launch_loop_tan_finish(const InstDesc& d, int n, hipStream_t s) {
+    const int ept = (d.T.N + kStageThreads - 1) / kStageThreads;
+#define DMFE_LTF(E, KK) hipLaunchKernelGGL((loop_tan_finish_kernel<E, KK>), dim3(1), dim3(kStageThreads), 0, s, d, n)
+#define DMFE_LTFE(KK)                       \
+    if (ept <= 1) DMFE_LTF(1, KK);          \
+    else if (ept <= 2) DMFE_LTF(2, KK);     \
+    else if (ept <= 4) DMFE_LTF(4, KK);     \
+    else DMFE_LTF(8, KK);
+    if (d.X.K == 1) { DMFE_LTFE(1) }
+    else { DMFE_LTFE(2) }
+#undef DMFE_LTFE
+#undef DMFE_LTF
 }
 
 void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t s) {
