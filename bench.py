@@ -53,6 +53,17 @@ def make_grid(N=512):
 PARAMS = dict(p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.6)
 
 
+USE_GRAPHS = True  # --no-graphs: the same launches issued directly (for tracers that cannot follow graphs)
+
+
+def new_context(g, prm, rows_hint):
+    import dynamite_amd as dm
+    ctx = dm.Context(g, prm, rows_hint=rows_hint)
+    if not USE_GRAPHS:
+        dm.set_loop_graphs(ctx, False)
+    return ctx
+
+
 def gpu_run(g, prep, steps, warmup):
     """The plain quench at L=512: `prep` steps from the quench, then `warmup` + `steps`
     timed steps of dmfe_run (the main loop with the step control on the device, replayed
@@ -61,7 +72,7 @@ def gpu_run(g, prep, steps, warmup):
     import dynamite_amd as dm
     prm = dm.default_params(p=PARAMS["p"], p2=PARAMS["p2"], lambda_=PARAMS["lam"], T0=PARAMS["T0"],
                             Gamma=PARAMS["Gamma"])
-    ctx = dm.Context(g, prm, rows_hint=prep + steps + warmup + 128)
+    ctx = new_context(g, prm, rows_hint=prep + steps + warmup + 128)
     ctx.init_quench()
     t0 = time.perf_counter()
     n, _ = ctx.run(1e300, prep)
@@ -148,7 +159,7 @@ def kernel_microbench(g, reps=200):
     st = synthetic_history(g.theta)
     prm = dm.default_params(p=PARAMS["p"], p2=PARAMS["p2"], lambda_=PARAMS["lam"], T0=PARAMS["T0"],
                             Gamma=PARAMS["Gamma"])
-    ctx = dm.Context(g, prm, rows_hint=len(st["t1grid"]) + 8)
+    ctx = new_context(g, prm, rows_hint=len(st["t1grid"]) + 8)
     ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
     ctx.interpolate(dm.POS_BSEARCH)
     ctx.synchronize()
@@ -182,7 +193,7 @@ def dense_microbench(N=512, order=9, B=4, reps=10):
     st = synthetic_history(g.theta)
     prm = dm.default_params(p=PARAMS["p"], p2=PARAMS["p2"], lambda_=PARAMS["lam"], T0=PARAMS["T0"],
                             Gamma=PARAMS["Gamma"])
-    ctx = dm.Context(g, prm, rows_hint=len(st["t1grid"]) + 8)
+    ctx = new_context(g, prm, rows_hint=len(st["t1grid"]) + 8)
     ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
     ctx.interpolate(dm.POS_BSEARCH)
     QK, QR = st["QK"].reshape(-1, N), st["QR"].reshape(-1, N)
@@ -211,14 +222,14 @@ def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
 
     def prm(G=0.7, lam=0.5):
         return dm.default_params(p=3, p2=4, lambda_=lam, T0=1e50, Gamma=G)
-    c = dm.Context(g, prm(), rows_hint=1024)
+    c = new_context(g, prm(), rows_hint=1024)
     c.init_quench()
     t0 = time.perf_counter()
     n_fwd, _ = c.run(tmax, 10**12)
     c.synchronize()
     el_fwd = time.perf_counter() - t0
     c.close()
-    c = dm.Context(g, prm(), rows_hint=1024)
+    c = new_context(g, prm(), rows_hint=1024)
     t0 = time.perf_counter()
     grad, t_final, n_grad = c.gradients(tmax, 10**12)
     c.synchronize()
@@ -227,7 +238,7 @@ def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
     log(f"[bench] sensitivities: {n_grad} steps to t={t_final:.4g}: forward {el_fwd:.2f}s, with tangents "
         f"{el_grad:.2f}s ({el_grad / el_fwd:.2f}x); grad {grad}")
     # finite-difference check on a shorter horizon with the dt sequence frozen
-    c = dm.Context(g, prm(), rows_hint=1024)
+    c = new_context(g, prm(), rows_hint=1024)
     c.init_quench()
     c.tangent_init([1.0, 0.0], [0.0, 1.0])
     seq, t = [], 0.0
@@ -240,7 +251,7 @@ def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
     J = np.array([[dE[0], dE[1]], [dQ[0], dQ[1]]])
 
     def replay(G, lam):
-        r = dm.Context(g, prm(G, lam), rows_hint=len(seq) + 64)
+        r = new_context(g, prm(G, lam), rows_hint=len(seq) + 64)
         r.init_quench()
         for dt, method in seq:
             r.set_control(dt, 1.0, 0.0, 10, method)
@@ -260,7 +271,7 @@ def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
     t0 = time.perf_counter()
     seq_grads = []
     for p in pl:
-        c = dm.Context(g, p, rows_hint=1024)
+        c = new_context(g, p, rows_hint=1024)
         seq_grads.append(c.gradients(many_tmax, 10**12)[0])
         c.close()
     el_seq = time.perf_counter() - t0
@@ -313,17 +324,17 @@ def batch_run(prep, steps, looped_steps, chunk=16):
     b = dm.Batch(gl, pl, rows_cap=prep + steps + 64)
     b.init_quench()
     t0 = time.perf_counter()
-    b.run(prep, use_graph=True, chunk=chunk)
+    b.run(prep, use_graph=USE_GRAPHS, chunk=chunk)
     log(f"[bench] batch of {n}: {prep} prep steps in {time.perf_counter() - t0:.2f}s, "
         f"T1 {min(b.T1(i) for i in range(n))}..{max(b.T1(i) for i in range(n))}")
     # the batch autotuner picks the lane-group width of the batch RHS on these states
     G_b, table_b = b.autotune(reps=4)
     log(f"[bench] batch autotune: G={G_b}; ms per RHS launch {table_b}")
-    b.run(2 * chunk, use_graph=True, chunk=chunk)  # warm-up, graph instantiation
+    b.run(2 * chunk, use_graph=USE_GRAPHS, chunk=chunk)  # warm-up, graph instantiation
     loops0 = np.array([b.get_control(i)["loop"] for i in range(n)])
     states = [(b.get_state(i), b.get_control(i)) for i in range(n)]
     t0 = time.perf_counter()
-    nb = b.run(steps, use_graph=True, chunk=chunk)
+    nb = b.run(steps, use_graph=USE_GRAPHS, chunk=chunk)
     el = time.perf_counter() - t0
     loops1 = np.array([b.get_control(i)["loop"] for i in range(n)])
     inst_steps = int((loops1 - loops0).sum())
@@ -338,7 +349,7 @@ def batch_run(prep, steps, looped_steps, chunk=16):
     ctxs, tuned = [], {}
     for i in range(n):
         st, c = states[i]
-        ctx = dm.Context(gl[i], pl[i], rows_hint=len(st["t1grid"]) + 3 * looped_steps + 64)
+        ctx = new_context(gl[i], pl[i], rows_hint=len(st["t1grid"]) + 3 * looped_steps + 64)
         ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
         ctx.set_control(c["delta_t"], c["delta"], c["delta_old"], c["loop"], c["method"])
         ctx.interpolate(dm.POS_BSEARCH)
@@ -408,9 +419,13 @@ def main():
     ap.add_argument("--looped-steps", type=int, default=48)
     ap.add_argument("--batch-only", action="store_true", help="only the batch-64 measurement")
     ap.add_argument("--sensitivity-only", action="store_true", help="only the sensitivity measurement")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="issue the step launches directly instead of replaying hipGraphs (same kernels)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain HBM copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args()
+    global USE_GRAPHS
+    USE_GRAPHS = not a.no_graphs
     if a.copy_probe > 0:
         import dynamite_amd as dm
         nb = int(a.copy_probe * 2**20)

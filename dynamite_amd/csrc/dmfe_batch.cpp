@@ -876,9 +876,13 @@ extern "C" int dmfe_batch_autotune(dmfe_batch* b, int reps, int* G_best, int32_t
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("dmfe_batch_autotune: ") + hipGetErrorString(e));
-    size_t qb = 0;
+    size_t qb = 0, qd = 0;
     for (size_t q = 1; q < cand.size(); ++q)
         if (ms[q] < ms[qb]) qb = q;
+    // the default width stays unless a candidate beats it by more than 2% (timing noise)
+    for (size_t q = 0; q < cand.size(); ++q)
+        if (cand[q] == rhs_default_G(b->maxm)) qd = q;
+    if (ms[qd] <= 1.02 * ms[qb]) qb = qd;
     int rc = dmfe_batch_set_lanes(b, cand[qb]);
     if (rc) return rc;
     if (G_best) *G_best = cand[qb];

@@ -1344,9 +1344,13 @@ extern "C" int dmfe_autotune(dmfe_ctx* c, int reps, int install, dmfe_rhs_shape*
             }
             ms[q] = std::min(ms[q], t);
         }
-    size_t qb = 0;
+    size_t qb = 0, qd = 0;
     for (size_t q = 1; q < cand.size(); ++q)
         if (ms[q] < ms[qb]) qb = q;
+    // the default shape stays unless a candidate beats it by more than 2% (timing noise)
+    for (size_t q = 0; q < cand.size(); ++q)
+        if (cand[q].G == rhs_default_G(c->m) && cand[q].split == rhs_default_split(c->N)) qd = q;
+    if (ms[qd] <= 1.02 * ms[qb]) qb = qd;
     c->G = cand[qb].G;
     c->split = cand[qb].split;
     if (install) rhs_shape_store(c->N, c->m, c->params.log_response, cand[qb]);

@@ -289,7 +289,8 @@ int dmfe_tuning_get(int64_t N, int m, int log_response, dmfe_rhs_shape* out);
 int dmfe_tuning_set(int64_t N, int m, int log_response, const dmfe_rhs_shape* shape);
 /* Autotuner: times `reps` back-to-back RHS launches for every valid (G, split) on the
  * context's current interpolated history (t > 0; best of two sweeps), keeps the fastest
- * on the context and, if `install`, stores it in the table for the context's class.
+ * (the default shape when it is within 2% of the fastest) on the context and, if
+ * `install`, stores it in the table for the context's class.
  * tried/tried_ms (may be NULL) receive up to max_tried candidates and their average
  * launch times in ms; *n_tried = number of candidates. */
 int dmfe_autotune(dmfe_ctx* ctx, int reps, int install, dmfe_rhs_shape* best, dmfe_rhs_shape* tried,
@@ -363,7 +364,8 @@ int dmfe_batch_records(dmfe_batch* b, int i, double* out);
  * dmfe_rhs_shape; each instance keeps the split of its class in the shape table).
  * dmfe_batch_autotune times `reps` launches of the batch RHS on the current states for
  * every valid G (best of two sweeps, after at least one batch step), keeps the fastest
- * and returns it with the candidates and their average launch times (ms). */
+ * (the default width when within 2% of it) and returns it with the candidates and their
+ * average launch times (ms). */
 int dmfe_batch_get_lanes(const dmfe_batch* b, int* G);
 int dmfe_batch_set_lanes(dmfe_batch* b, int G);
 int dmfe_batch_autotune(dmfe_batch* b, int reps, int* G_best, int32_t* tried_G, double* tried_ms, int max_tried,
