@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_tan_kernel(Tables T, His
 }
 
 // ---- finish: RHS epilogue + RK stage -------------------------------------------------
-template <int EPT>
+template <int EPT, bool PRE = true>
 __device__ __forceinline__ void finish_block(const Tables& T, const History& H, const Work& W, const Model& M, const FinishArgs& a,
                              double* red /* 8 * kStageThreads / 64 */) {
     const int N = T.N;
@@ -1020,6 +1020,34 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
     const bool live = t > 0;
     const int* Fl = W.Floor + (size_t)a.fbuf * N;
     const double* Df = W.diff + (size_t)a.fbuf * N;
+    // operands of the stage combination (earlier stages' RHS, the step's start row) do not
+    // depend on this stage's RHS: they are requested first, so that their latency
+    // overlaps the RHS epilogue and the reductions below
+    // (earlier stages' RHS values are held in registers for EPT <= 2 only, and not in the
+    // many-instance batch, whose stage launch is throughput-bound and keeps occupancy)
+    constexpr bool kHold = PRE && EPT <= 2;
+    constexpr int kPre = kHold ? kMaxStages : 1;
+    double2 g0v[EPT];
+    double hpk[EPT][kPre], hpr[EPT][kPre];
+    if (a.mode != 0) {
+        const double2* g0 = H.V + (size_t)a.g0row * N;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int j = threadIdx.x + e * kStageThreads;
+            g0v[e] = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < kPre; ++q) hpk[e][q] = hpr[e][q] = 0.0;
+            if (j >= N) continue;
+            g0v[e] = g0[j];
+#pragma unroll
+            for (int q = 0; q < kPre; ++q) {
+                if (kHold && q < n && q < a.nterms) {
+                    hpk[e][q] = W.hK[(size_t)q * N + j];
+                    hpr[e][q] = W.hR[(size_t)q * N + j];
+                }
+            }
+        }
+    }
     const double s1 = fl1(M, qK0) / M.T0, s2 = -rint_at(H, Fl, Df, N - 1, a.lr + 1, live), s3 = qK0 / M.T0,
                  s4 = 2 * M.Gamma;
     double hk[EPT], hr[EPT];
@@ -1078,13 +1106,17 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
             if (j == 0) { dsK0 = dsK; sK0 = sK; }
         }
         block_sum<6, kStageThreads>(v, red);
-        if (threadIdx.x == 0)
-            W.scal[0] = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + (dsK0 * qK0 + sK0 * hk[0]) / M.T0;
+        if (threadIdx.x == 0) {
+            const double d = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + (dsK0 * qK0 + sK0 * hk[0]) / M.T0;
+            W.scal[0] = d;
+            red[0] = d;  // block_sum is done with red
+        }
         __syncthreads();
     }
-    const double dr = W.scal[0];
+    double dr;
+    if (n == 0) dr = red[0];  // LDS and global reads kept apart (no generic-pointer load)
+    else dr = W.scal[0];
     // stage combination g_{n+1} = g_0 + sum_j h_j * (dt a_{n+1,j}) and the row write
-    const double2* g0 = H.V + (size_t)a.g0row * N;
     double2* wv = H.V + (size_t)a.wrow * N;
     double2* wd = H.D + (size_t)a.wrow * N;
     double rr[2] = {0.0, 0.0};
@@ -1094,19 +1126,26 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
     for (int e = 0; e < EPT; ++e) {
         const int j = threadIdx.x + e * kStageThreads;
         if (j >= N) continue;
-        const double2 g = g0[j];
+        const double2 g = g0v[e];
         double gK = g.x, gR = g.y;
         // static trip count: a.coef stays in registers when `a` is a local copy
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int q = 0; q < kMaxStages; ++q) {
-            if (q >= a.nterms) break;
-            const double hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
-            const double hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
-            gK = gK + hq * a.coef[q];
-            gR = gR + hqr * a.coef[q];
+            if (q < a.nterms) {
+                double hq, hqr;
+                if constexpr (kHold) {
+                    hq = (q == n) ? hk[e] : hpk[e][q];
+                    hqr = (q == n) ? hr[e] : hpr[e][q];
+                } else {
+                    hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
+                    hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
+                }
+                gK = gK + hq * a.coef[q];
+                gR = gR + hqr * a.coef[q];
+            }
         }
-        const double h0k = (n == 0) ? hk[e] : W.hK[j];
-        const double h0r = (n == 0) ? hr[e] : W.hR[j];
+        const double h0k = (n == 0) ? hk[e] : (kHold ? hpk[e][0] : W.hK[j]);
+        const double h0r = (n == 0) ? hr[e] : (kHold ? hpr[e][0] : W.hR[j]);
         wv[j] = make_double2(gK, gR);
         wd[j] = make_double2(a.tdiff * h0k, a.tdiff * h0r);
         // rstep on the new row (time_steps.cpp:83-92)
@@ -1117,13 +1156,20 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
         if (j == 0) { gK0 = gK; sKg0 = sK; }
         if (a.last) {
             double eK = g.x, eR = g.y;
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int q = 0; q < kMaxStages; ++q) {
-                if (q > n) break;
-                const double hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
-                const double hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
-                eK = eK + hq * a.coefE[q];
-                eR = eR + hqr * a.coefE[q];
+                if (q <= n) {
+                    double hq, hqr;
+                    if constexpr (kHold) {
+                        hq = (q == n) ? hk[e] : hpk[e][q];
+                        hqr = (q == n) ? hr[e] : hpr[e][q];
+                    } else {
+                        hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
+                        hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
+                    }
+                    eK = eK + hq * a.coefE[q];
+                    eR = eR + hqr * a.coefE[q];
+                }
             }
             err[0] += fabs(gK - eK) + fabs(gR - eR);
         }
@@ -1419,12 +1465,12 @@ __global__ void loop_end_kernel(InstDesc d) {
 
 // Stage launch n of a batch step: map[b] = (instance, role); role -2 finish of stage
 // n-1, -1 theta-row search, k >= 0 phi2 row k of the interpolation of stage n.
-template <int EPT>
+template <int EPT, bool PRE>
 __device__ __forceinline__ void stage_role_body(const InstDesc& d, int role, int n, double* lds, double* red) {
     const Ctl& c = *d.ctl;
     if (!c.active || n > c.S) return;
     if (role == -2) {
-        if (n >= 1) finish_block<EPT>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
+        if (n >= 1) finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
         return;
     }
     const SearchArgs& a = c.srch[n];
@@ -1440,7 +1486,7 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(const InstDe
     __shared__ double red[8 * (kStageThreads / kWave)];
     const int2 mp = map[blockIdx.x];
     if (mp.x < 0) return;
-    stage_role_body<EPT>(D[mp.x], mp.y, n, lds, red);
+    stage_role_body<EPT, false>(D[mp.x], mp.y, n, lds, red);
 }
 
 // device loop of one context: block 0 finish, block 1 theta row, block 2 + k phi2 row k
@@ -1448,7 +1494,7 @@ template <int EPT>
 __global__ __launch_bounds__(kStageThreads) void loop_stage_kernel(InstDesc d, int n) {
     __shared__ double lds[EPT * kStageThreads];
     __shared__ double red[8 * (kStageThreads / kWave)];
-    stage_role_body<EPT>(d, (int)blockIdx.x - 2, n, lds, red);
+    stage_role_body<EPT, true>(d, (int)blockIdx.x - 2, n, lds, red);
 }
 
 // RHS launch n: map[b] = (instance, row j, chunk, -); blocks of one (j, chunk) for all
