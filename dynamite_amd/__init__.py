@@ -367,6 +367,11 @@ def gradients_many(grids, params, tmax, max_loop, rows_hint=1024):
     return grads, tf, st
 
 
+def set_rollback(ctx, on):
+    """The reference GPU loop's rollback on error spikes (off by default, as on the CPU path)."""
+    _check(lib().dmfe_set_rollback(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_rollback")
+
+
 def set_loop_graphs(ctx, on):
     """dmfe_run replays captured graphs (True, default) or launches directly (False)."""
     _check(lib().dmfe_set_loop_graphs(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_loop_graphs")

@@ -129,6 +129,8 @@ struct dmfe_ctx {
     int G = 8;                // lanes per B2 entry of the RHS kernel
     dmfe_batch* dloop = nullptr;  // dmfe_run's device loop (launch maps, captured graphs)
     bool loop_graphs = true;      // dmfe_run replays captured graphs (dmfe_set_loop_graphs)
+    bool rollback = false;        // the reference GPU loop's rollback (dmfe_set_rollback)
+    int64_t last_rollback = -1000;  // last_rollback_loop (simulation_runner.cpp:53)
     bool dense_mfma = false;  // dense (B-spline) grid: A1/A2 contracted on MFMA, weights entry-major
     // forward-mode sensitivities (dmfe_tangent_init): tangent histories and work
     int tanK = 0;
@@ -529,8 +531,8 @@ int loop_iteration(dmfe_ctx* c, double* t_out) {
         }
         rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
         if (rc) return rc;
-        // last_rollback_loop stays -1000 on the CPU path (simulation_runner.cpp:53,220)
-        if (c->delta < dmax / 2 && c->loop - (-1000) > 1000) {
+        // last_rollback_loop stays -1000 unless rollbacks are on (simulation_runner.cpp:53,220)
+        if (c->delta < dmax / 2 && c->loop - c->last_rollback > 1000) {
             c->dt *= 0.5;
             c->method = DMFE_SSPRK104;
         }
@@ -546,7 +548,29 @@ int loop_iteration(dmfe_ctx* c, double* t_out) {
         c->method = DMFE_RK54;
         c->dt *= 0.5;
     }
-    if (t_out) *t_out = t;
+    double t_loop = t;
+    if (c->rollback && c->delta > 2 * dmax) {
+        // the GPU main loop's rollback (simulation_runner.cpp:280-303): drop the last n <= 10
+        // steps (rollbackState, simulation_control.cpp:14-76: truncate the history, delta_t
+        // from the kept last step, loop -= n, interpolate), then halve the step and restart
+        // with SSPRK104
+        const int64_t n = std::min<int64_t>(10, std::max<int64_t>(0, c->T1 - 2));
+        if (n > 0) {
+            c->T1 -= n;
+            c->t1.resize((size_t)c->T1);
+            c->rho.resize((size_t)c->T1);
+            c->dt = c->t1[c->T1 - 1] - c->t1[c->T1 - 2];
+            c->loop -= n;
+            c->interp_valid = false;
+            rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+            if (rc) return rc;
+            c->last_rollback = c->loop;
+            t_loop = c->t1[c->T1 - 1] + c->dt;
+        }
+        c->dt = 0.5 * std::min(std::max(c->dt, c->params.delta_t_min), rmax[c->method - 1] / c->specRad);
+        c->method = DMFE_SSPRK104;
+    }
+    if (t_out) *t_out = t_loop;
     return DMFE_OK;
 }
 
@@ -968,6 +992,12 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
     return rc;
 }
 
+extern "C" int dmfe_set_rollback(dmfe_ctx* c, int on) {
+    if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_rollback: null context");
+    c->rollback = on != 0;
+    return DMFE_OK;
+}
+
 extern "C" int dmfe_set_loop_graphs(dmfe_ctx* c, int on) {
     if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_loop_graphs: null context");
     c->loop_graphs = on != 0;
@@ -976,8 +1006,9 @@ extern "C" int dmfe_set_loop_graphs(dmfe_ctx* c, int on) {
 
 extern "C" int dmfe_run(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps) {
     if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_run: no state");
-    // dense MFMA contexts and per-launch profiling keep the host-driven loop
-    if (!c->dense_mfma && !c->profiling) return run_device_loop(c, tmax, max_loop, corr, corr_cap, steps);
+    // dense MFMA contexts, per-launch profiling and rollbacks keep the host-driven loop
+    if (!c->dense_mfma && !c->profiling && !c->rollback)
+        return run_device_loop(c, tmax, max_loop, corr, corr_cap, steps);
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];
     const double eps = std::numeric_limits<double>::epsilon();
@@ -1107,7 +1138,7 @@ extern "C" int dmfe_gradients(dmfe_ctx* c, double tmax, int64_t max_loop, double
     if (rc) return rc;
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];
-    if (!c->dense_mfma && !c->profiling) {
+    if (!c->dense_mfma && !c->profiling && !c->rollback) {
         rc = run_device_loop(c, tmax, max_loop, nullptr, 0, &done, &t);
         if (rc) return rc;
     } else {

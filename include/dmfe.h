@@ -208,6 +208,16 @@ int dmfe_loop_iteration(dmfe_ctx* ctx, double* t_out);
  * on dense grids and contexts with per-launch profiling on use the host loop. */
 int dmfe_run(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps);
 
+/* The rollback of the reference's GPU main loop (simulation_runner.cpp:280-303,
+ * rollbackState simulation_control.cpp:14-76), off by default as on the CPU path that
+ * the results are checked against: when a step ends with delta > 2 delta_max, the last
+ * n = min(10, T1 - 2) steps are dropped (history truncated, delta_t = the kept last
+ * step, loop -= n, re-interpolation), then delta_t = 0.5 min(max(delta_t,
+ * delta_t_min), rmax[method] / specRad) and the integrator restarts as SSPRK104; the
+ * maintenance's dt halving then waits 1000 loops after a rollback.  A context with
+ * rollbacks on runs dmfe_run on the host-driven loop. */
+int dmfe_set_rollback(dmfe_ctx* ctx, int on);
+
 /* dmfe_run replays captured hipGraphs of its 16-step launch sequence (on, default) or
  * issues the same launches directly (off).  No counterpart in the reference.  Turn it
  * off for contexts that host threads drive concurrently: a stream capture in one

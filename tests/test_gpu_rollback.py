@@ -1,0 +1,83 @@
+"""The reference GPU main loop's rollback (simulation_runner.cpp:280-303, rollbackState
+simulation_control.cpp:14-76), an opt-in of dmfe_set_rollback.
+
+No CPU oracle covers it (the CPU path never rolls back), so it is pinned by what the
+reference code fixes: a step whose error exceeds 2 delta_max (forced here by a 1000x step)
+drops the last min(10, T1 - 2) rows, which must equal the earlier history bit for bit;
+delta_t becomes 0.5 min(max(t_last - t_prev, delta_t_min), rmax[method] / specRad),
+the loop counter goes back by n and the integrator restarts as SSPRK104; the context
+then continues exactly like a fresh context given the truncated state and control.
+"""
+import numpy as np
+import pytest
+
+import dynamite_amd as dm
+
+pytestmark = pytest.mark.gpu
+
+
+def setup():
+    g = dm.generate_grid(64, 1e2, 5, dm.INTERP_POLY, 9, 0)
+    p = dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.8)
+    return g, p
+
+
+def test_rollback_drops_steps_and_restarts():
+    g, p = setup()
+    ctx = dm.Context(g, p, rows_hint=256)
+    ctx.init_quench()
+    for _ in range(80):
+        ctx.loop_iteration()
+    before, ctl = ctx.get_state(), ctx.get_control()
+    T1 = len(before["t1grid"])
+    dm.set_rollback(ctx, True)
+    ctx.set_control(1000 * ctl["delta_t"], ctl["delta"], ctl["delta_old"], ctl["loop"], ctl["method"])
+    t = ctx.loop_iteration()
+    after, c2 = ctx.get_state(), ctx.get_control()
+    assert c2["delta"] > 2 * p.delta_max  # the forced step did fail
+    n = min(10, T1 + 1 - 2)
+    assert len(after["t1grid"]) == T1 + 1 - n
+    k = T1 + 1 - n
+    for key in ("t1grid", "dtratio", "rvec", "drvec"):
+        assert np.array_equal(after[key], before[key][:k]), key
+    for key in ("QK", "QR", "dQK", "dQR"):
+        assert np.array_equal(after[key], before[key][:k * 64]), key
+    assert c2["loop"] == ctl["loop"] + 1 - n
+    tl = before["t1grid"]
+    last = tl[k - 1] - tl[k - 2]
+    specRad = 4 * np.sqrt(3 * 2 * 0.5 + 4 * 3 * 0.5)  # 4 sqrt(f''(1)), p=3, q=4, lambda=0.5
+    assert c2["delta_t"] == 0.5 * min(max(last, p.delta_t_min), p.rmax[ctl["method"] - 1] / specRad)
+    assert c2["method"] == dm.SSPRK104
+    assert t == tl[k - 1] + last
+    # the positions were recomputed: same fields as a fresh context on the truncated state
+    ref = dm.Context(g, p, rows_hint=256)
+    ref.set_state(after["t1grid"], after["QK"], after["QR"], after["dQK"], after["dQR"], after["rvec"],
+                  after["drvec"], after["dtratio"])
+    ref.interpolate(dm.POS_BSEARCH)
+    fa, fr = ctx.fields(), ref.fields()
+    for name in dm.FIELD_NAMES:
+        assert np.array_equal(fa[name], fr[name]), name
+    # and the run goes on like the fresh context with the same control
+    ref.set_control(c2["delta_t"], c2["delta"], c2["delta_old"], c2["loop"], c2["method"])
+    for _ in range(25):
+        ctx.loop_iteration()
+        ref.loop_iteration()
+    s1, s2 = ctx.get_state(), ref.get_state()
+    for key in ("t1grid", "QK", "QR", "rvec"):
+        assert np.array_equal(s1[key], s2[key]), key
+    ctx.close()
+    ref.close()
+
+
+def test_rollback_off_by_default():
+    g, p = setup()
+    ctx = dm.Context(g, p, rows_hint=256)
+    ctx.init_quench()
+    for _ in range(40):
+        ctx.loop_iteration()
+    c = ctx.get_control()
+    T1 = ctx.T1
+    ctx.set_control(1000 * c["delta_t"], c["delta"], c["delta_old"], c["loop"], c["method"])
+    ctx.loop_iteration()
+    assert ctx.T1 == T1 + 1 and ctx.get_control()["loop"] == c["loop"] + 1
+    ctx.close()
