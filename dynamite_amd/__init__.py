@@ -367,6 +367,11 @@ def gradients_many(grids, params, tmax, max_loop, rows_hint=1024):
     return grads, tf, st
 
 
+def set_serk2(ctx, on):
+    """The reference GPU loop's SERK2 promotion at the maintenance (off by default)."""
+    _check(lib().dmfe_set_serk2(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_serk2")
+
+
 def set_rollback(ctx, on):
     """The reference GPU loop's rollback on error spikes (off by default, as on the CPU path)."""
     _check(lib().dmfe_set_rollback(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_rollback")

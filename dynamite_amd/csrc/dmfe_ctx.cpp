@@ -130,6 +130,7 @@ struct dmfe_ctx {
     dmfe_batch* dloop = nullptr;  // dmfe_run's device loop (launch maps, captured graphs)
     bool loop_graphs = true;      // dmfe_run replays captured graphs (dmfe_set_loop_graphs)
     bool rollback = false;        // the reference GPU loop's rollback (dmfe_set_rollback)
+    bool use_serk2 = false;       // the GPU loop's SERK2 promotion at the maintenance (dmfe_set_serk2)
     int64_t last_rollback = -1000;  // last_rollback_loop (simulation_runner.cpp:53)
     bool dense_mfma = false;  // dense (B-spline) grid: A1/A2 contracted on MFMA, weights entry-major
     // forward-mode sensitivities (dmfe_tangent_init): tangent histories and work
@@ -345,7 +346,10 @@ int copy_out(dmfe_ctx* c, double* dst, const void* src, size_t bytes) {
 // step: [search(0) if T1 == 1] rhs(0) | finish(0)+search(1) | rhs(1) | ... |
 // rhs(S-1) | finish(S-1)+final search; the host replays the scalar time
 // bookkeeping (gt, t1grid, delta_t_ratio) in the reference's FP64 order.
+int rk_step_serk2(dmfe_ctx* c, int q, double* err_out);
+
 int rk_step(dmfe_ctx* c, double* err_out) {
+    if (c->method >= 3) return rk_step_serk2(c, 2 * (c->method - 2), err_out);
     const bool ss = (c->method != DMFE_RK54);
     const Tableau& tb = ss ? ssprk104() : dp54();
     const int S = tb.S;
@@ -534,7 +538,11 @@ int loop_iteration(dmfe_ctx* c, double* t_out) {
         // last_rollback_loop stays -1000 unless rollbacks are on (simulation_runner.cpp:53,220)
         if (c->delta < dmax / 2 && c->loop - c->last_rollback > 1000) {
             c->dt *= 0.5;
-            c->method = DMFE_SSPRK104;
+            // CPU path: SSPRK104.  The GPU loop with use_serk2: RK54 -> SSPRK104, else one
+            // SERK2 level up (init_SERK2(2 (init - 1)) sets init + 1, runge_kutta.cu:274),
+            // capped at the rmax table
+            if (!c->use_serk2 || c->method == DMFE_RK54) c->method = DMFE_SSPRK104;
+            else if (c->method < kMaxMethod) c->method += 1;
         }
     }
     const double* rmax = c->params.rmax;
@@ -568,7 +576,8 @@ int loop_iteration(dmfe_ctx* c, double* t_out) {
             t_loop = c->t1[c->T1 - 1] + c->dt;
         }
         c->dt = 0.5 * std::min(std::max(c->dt, c->params.delta_t_min), rmax[c->method - 1] / c->specRad);
-        c->method = DMFE_SSPRK104;
+        // a SERK2 level above the first steps down, anything else restarts as SSPRK104
+        c->method = c->method > 3 ? c->method - 1 : DMFE_SSPRK104;
     }
     if (t_out) *t_out = t_loop;
     return DMFE_OK;
@@ -598,6 +607,7 @@ void free_ctx(dmfe_ctx* c) {
     hipFree(c->W.Floor); hipFree(c->W.diff); hipFree(c->W.rInt); hipFree(c->W.partial);
     hipFree(c->W.qkb1c0); hipFree(c->W.hK); hipFree(c->W.hR); hipFree(c->W.scal);
     hipFree(c->W.fA[0]); hipFree(c->W.fA[1]);
+    hipFree(c->W.sE); hipFree(c->W.sF);
     tangent_free(c);
     hipFree(c->fields);
     if (c->pinned) hipHostFree(c->pinned);
@@ -675,6 +685,8 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
          hipMalloc(&c->W.qkb1c0, bN) == hipSuccess && hipMalloc(&c->W.hK, bN * kMaxStages) == hipSuccess &&
          hipMalloc(&c->W.hR, bN * kMaxStages) == hipSuccess && hipMalloc(&c->W.scal, 8 * sizeof(double)) == hipSuccess &&
          hipHostMalloc((void**)&c->pinned, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipMalloc(&c->W.sE, (size_t)N * sizeof(double2)) == hipSuccess &&
+         hipMalloc(&c->W.sF, (size_t)N * sizeof(double2)) == hipSuccess;
     if (ok && c->dense_mfma)
         ok = hipMalloc(&c->W.fA[0], (size_t)P * sizeof(double2)) == hipSuccess &&
              hipMalloc(&c->W.fA[1], (size_t)P * sizeof(double2)) == hipSuccess;
@@ -813,7 +825,9 @@ extern "C" int dmfe_get_state(dmfe_ctx* c, double* t1grid, double* QK, double* Q
 }
 
 extern "C" int dmfe_set_control(dmfe_ctx* c, double delta_t, double delta, double delta_old, int64_t loop, int method) {
-    if (!c || (method != DMFE_RK54 && method != DMFE_SSPRK104)) return set_error(DMFE_ERR_ARG, "dmfe_set_control: bad argument");
+    if (!c || method < DMFE_RK54 || method > kMaxMethod)
+        return set_error(DMFE_ERR_ARG, "dmfe_set_control: bad argument (method 1 RK54, 2 SSPRK104, 3.." +
+                                           std::to_string(kMaxMethod) + " SERK2(2 (method - 2)))");
     c->dt = delta_t; c->delta = delta; c->delta_old = delta_old; c->loop = loop; c->method = method;
     return DMFE_OK;
 }
@@ -992,6 +1006,170 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
     return rc;
 }
 
+// ---- SERK2 (the reference GPU loop's stabilized integrator) --------------------------
+// Coefficients (runge_kutta.cu:228-270): s = 2q stages with Chebyshev stage polynomials
+// T_i(x) T_2(x)^(j-1), x = 1 + z q / s^2; b[0..s] make sum_k b_k P_k(z) match
+// R(z) = T_s(w0 + w1 z) / T_s(w0) at z = 1..s+1, w0 = 1 + (19/20) / s^2,
+// w1 = T_s(w0) / (s U_{s-1}(w0)); solved in long double by Gaussian elimination with
+// partial pivoting.
+namespace {  // the same unnamed namespace as the step functions
+long double cheb_T(int n, long double x) {
+    long double a = 1.0L, b = x;
+    if (n == 0) return a;
+    for (int k = 1; k < n; ++k) {
+        const long double c2 = 2.0L * x * b - a;
+        a = b;
+        b = c2;
+    }
+    return b;
+}
+long double cheb_U(int n, long double x) {
+    long double a = 1.0L, b = 2.0L * x;
+    if (n == 0) return a;
+    for (int k = 1; k < n; ++k) {
+        const long double c2 = 2.0L * x * b - a;
+        a = b;
+        b = c2;
+    }
+    return b;
+}
+
+std::vector<double> serk2_coeffs(int q) {
+    const int m = 2, s = m * q, n = s + 1;
+    const long double a2 = (long double)q / ((long double)s * s);
+    const long double w0 = 1.0L + (19.0L / 20.0L) / ((long double)s * s);
+    const long double w1 = cheb_T(s, w0) / ((long double)s * cheb_U(s - 1, w0));
+    std::vector<std::vector<long double>> A((size_t)n, std::vector<long double>((size_t)n + 1, 0.0L));
+    for (int r = 0; r < n; ++r) {
+        const long double z = (long double)(r + 1), x = 1.0L + a2 * z;
+        auto& row = A[(size_t)r];
+        row[0] = 1.0L;
+        long double tm = 1.0L;
+        for (int j = 1; j <= q; ++j) {
+            for (int i = 1; i <= m; ++i) row[(size_t)(i + m * (j - 1))] = cheb_T(i, x) * tm;
+            tm *= cheb_T(m, x);
+        }
+        row[(size_t)n] = cheb_T(s, w0 + w1 * z) / cheb_T(s, w0);
+    }
+    for (int k = 0; k < n; ++k) {
+        int pr = k;
+        for (int r = k + 1; r < n; ++r)
+            if (std::fabs(A[(size_t)r][(size_t)k]) > std::fabs(A[(size_t)pr][(size_t)k])) pr = r;
+        std::swap(A[(size_t)k], A[(size_t)pr]);
+        for (int r = k + 1; r < n; ++r) {
+            const long double f = A[(size_t)r][(size_t)k] / A[(size_t)k][(size_t)k];
+            for (int col = k; col <= n; ++col) A[(size_t)r][(size_t)col] -= f * A[(size_t)k][(size_t)col];
+        }
+    }
+    std::vector<double> b((size_t)n);
+    for (int k = n - 1; k >= 0; --k) {
+        long double v = A[(size_t)k][(size_t)n];
+        for (int col = k + 1; col < n; ++col) v -= A[(size_t)k][(size_t)col] * (long double)b[(size_t)col];
+        b[(size_t)k] = (double)(v / A[(size_t)k][(size_t)k]);
+    }
+    return b;
+}
+
+// One SERK2 step (SERK2GPU, runge_kutta.cu:397-487, on this path's searches): stage n
+// searches the positions (stage 0 only on a fresh history), evaluates the RHS and the
+// stage launch of n+1 finishes it: g += dt alpha h (even n), g = 2 g - g_e + 2 dt alpha h
+// (odd n, g_e := g), alpha = 1 / (4q); the embedded sum F = b_0 g_0 + sum b_{n+1} g_{n+1}
+// and the error sum |F - g_s| + |t + dt - gt_s| at the last stage.
+int rk_step_serk2(dmfe_ctx* c, int q, double* err_out) {
+    if (c->tanK) return set_error(DMFE_ERR_STATE, "sensitivities are not carried through SERK2 steps");
+    const int S = 2 * q;
+    const std::vector<double> b = serk2_coeffs(q);
+    int rc = ensure_rows(c, c->T1 + 1);
+    if (rc) return rc;
+    if (!c->interp_valid && c->T1 > 1) {
+        rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    const double dt = c->dt, alpha = 1.0 / (4 * q);
+    const int64_t g0row = c->T1 - 1;
+    const double t0 = c->t1[c->T1 - 1];
+    double gt = t0, gte = t0;
+    const double gtf = t0 + dt;
+    FinishArgs pending{};
+    bool have_pending = false;
+    for (int n = 0; n < S; ++n) {
+        SearchArgs sa{};
+        bool search = false;
+        if (c->T1 == 1 || n != 0) {
+            search = true;
+            sa = make_search(c, n == 0 ? kSearchB : kSearchI, nullptr, nullptr, nullptr, nullptr);
+        }
+        if (search || have_pending) {
+            rc = run_stage(c, search ? &sa : nullptr, have_pending ? &pending : nullptr);
+            if (rc) return rc;
+            have_pending = false;
+        }
+        rc = run_rhs_rows(c, nullptr);
+        if (rc) return rc;
+        FinishArgs& f = pending;
+        f = FinishArgs{};
+        f.mode = 1;
+        f.stage = n == 0 ? 0 : 1;
+        f.nterms = 1;
+        f.split = c->split;
+        f.lr = (int)(c->T1 - 1);
+        f.t_rhs = c->t1[c->T1 - 1];
+        f.serk = (n % 2 == 1) ? 2 : 1;
+        f.sa = (n % 2 == 1) ? 2 * dt * alpha : dt * alpha;
+        f.sb0 = b[0];
+        f.sb = b[(size_t)n + 1];
+        f.last = (n == S - 1) ? 1 : 0;
+        if (n == 0) {
+            gt = t0 + dt * alpha * 1.0;
+            c->t1.push_back(gt);
+            const size_t idx = c->t1.size() - 1;
+            const double tdiff = c->t1[idx] - c->t1[idx - 1];
+            const double r = (idx > 1) ? tdiff / (c->t1[idx - 1] - c->t1[idx - 2]) : 0.0;
+            c->rho.push_back(r);
+            f.g0row = (int)g0row;
+            f.wrow = (int)idx;
+            f.t_new = gt;
+            f.tdiff = tdiff;
+            f.rho_new = r;
+            c->T1 += 1;
+        } else {
+            if (n % 2 == 0) {
+                gt = gt + dt * alpha * 1.0;
+            } else {
+                gt = 2 * gt - gte + 2 * dt * alpha * 1.0;
+                gte = gt;
+            }
+            const size_t L = c->t1.size();
+            c->t1[L - 1] = gt;
+            const double tdiff = c->t1[L - 1] - c->t1[L - 2];
+            const double r = (L > 2) ? tdiff / (c->t1[L - 2] - c->t1[L - 3]) : 0.0;
+            c->rho[L - 1] = r;
+            f.g0row = (int)(L - 1);  // the previous stage's values
+            f.wrow = (int)(L - 1);
+            f.t_new = gt;
+            f.tdiff = tdiff;
+            f.rho_new = r;
+        }
+        if (f.last) f.gt_err = std::fabs(gtf - gt);
+        have_pending = true;
+    }
+    // positions for the next step, whose first stage reuses them
+    const SearchArgs fin = make_search(c, kSearchI, nullptr, nullptr, nullptr, nullptr);
+    rc = run_stage(c, &fin, &pending);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->pinned, c->W.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err_out) *err_out = c->pinned[0];
+    return DMFE_OK;
+}
+}  // namespace
+
+extern "C" int dmfe_set_serk2(dmfe_ctx* c, int on) {
+    if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_serk2: null context");
+    c->use_serk2 = on != 0;
+    return DMFE_OK;
+}
+
 extern "C" int dmfe_set_rollback(dmfe_ctx* c, int on) {
     if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_rollback: null context");
     c->rollback = on != 0;
@@ -1006,8 +1184,8 @@ extern "C" int dmfe_set_loop_graphs(dmfe_ctx* c, int on) {
 
 extern "C" int dmfe_run(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps) {
     if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_run: no state");
-    // dense MFMA contexts, per-launch profiling and rollbacks keep the host-driven loop
-    if (!c->dense_mfma && !c->profiling && !c->rollback)
+    // dense MFMA contexts, per-launch profiling, rollbacks and SERK2 keep the host-driven loop
+    if (!c->dense_mfma && !c->profiling && !c->rollback && !c->use_serk2 && c->method < 3)
         return run_device_loop(c, tmax, max_loop, corr, corr_cap, steps);
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];
@@ -1138,7 +1316,7 @@ extern "C" int dmfe_gradients(dmfe_ctx* c, double tmax, int64_t max_loop, double
     if (rc) return rc;
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];
-    if (!c->dense_mfma && !c->profiling && !c->rollback) {
+    if (!c->dense_mfma && !c->profiling && !c->rollback && !c->use_serk2 && c->method < 3) {
         rc = run_device_loop(c, tmax, max_loop, nullptr, 0, &done, &t);
         if (rc) return rc;
     } else {

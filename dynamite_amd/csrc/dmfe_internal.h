@@ -26,6 +26,7 @@ void grid_bspline_weights(const std::vector<long double>& x, const std::vector<l
 constexpr int kMaxN = 2048;     // LDS sizing of the row kernels
 constexpr int kMaxStages = 10;  // SSPRK(10,4)
 constexpr int kSums = 8;        // quadratures per RHS row
+constexpr int kMaxMethod = 11;  // integrators 1 RK54, 2 SSPRK104, 3.. SERK2 levels (the rmax table)
 // RHS rows with N <= kTailMaxN keep the last two history rows (V[T1-2], D[T1-1]) in LDS
 // beside V[T1-1], so B2 targets in the last time interval read the block's LDS copy.
 constexpr int kTailMaxN = 1024;
@@ -77,6 +78,8 @@ struct Work {
     double* hR;       // [kMaxStages][N]
     double* scal;     // [0] dr (drstep2), [1] error norm, [2] rstep / energy scratch
     double2* fA[2];   // dense grids: A1, A2 fields (QK, QR) of the current RHS [P], or null
+    double2* sE;      // [N] SERK2: the stage two steps back (g_e), or null
+    double2* sF;      // [N] SERK2: the embedded sum of b_k g_k, or null
 };
 
 enum SearchMode { kSearchB = 0, kSearchI = 1, kSearchSame = 2, kSearchCopy = 3 };
@@ -143,6 +146,11 @@ struct FinishArgs {
     double coefE[kMaxStages];  // dt * b2_j
     double gt_err;             // |gt_S - gte|
     int fbuf;                  // Floor/diff buffer of the interpolation this RHS used
+    // SERK2 stages (serk 1: g += sa h; 2: g = 2 g - g_e + sa h): the base row is g0row (the
+    // start row at stage 0, else the previous stage's row), sb0 / sb weight the start row /
+    // this stage in the embedded sum (W.sF); stage is 0 or 1 (hK slot)
+    int serk;
+    double sa, sb0, sb;
 };
 
 // Forward-mode sensitivities: K <= kMaxTan directions, direction k perturbs (Gamma, lambda)

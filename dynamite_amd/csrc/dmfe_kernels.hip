@@ -1128,20 +1128,62 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
         if (j >= N) continue;
         const double2 g = g0v[e];
         double gK = g.x, gR = g.y;
-        // static trip count: a.coef stays in registers when `a` is a local copy
+        double errj = 0.0;
+        if (a.serk) {
+            // SERK2 (runge_kutta.cu:397-487): g is the previous stage (the start row at the
+            // first stage); serk 1 an Euler step, 2 the Chebyshev step against W.sE
+            if (a.serk == 2) {
+                const double2 ge = W.sE[j];
+                gK = 2 * g.x - ge.x + hk[e] * a.sa;
+                gR = 2 * g.y - ge.y + hr[e] * a.sa;
+                W.sE[j] = make_double2(gK, gR);
+            } else {
+                gK = g.x + hk[e] * a.sa;
+                gR = g.y + hr[e] * a.sa;
+                if (n == 0) W.sE[j] = g;
+            }
+            double2 F = (n == 0) ? make_double2(g.x * a.sb0, g.y * a.sb0) : W.sF[j];
+            if (a.sb != 0.0) {
+                F.x = F.x + gK * a.sb;
+                F.y = F.y + gR * a.sb;
+            }
+            W.sF[j] = F;
+            if (a.last) errj = fabs(F.x - gK) + fabs(F.y - gR);
+        } else {
+            // static trip count: a.coef stays in registers when `a` is a local copy
 #pragma clang loop unroll(full)
-        for (int q = 0; q < kMaxStages; ++q) {
-            if (q < a.nterms) {
-                double hq, hqr;
-                if constexpr (kHold) {
-                    hq = (q == n) ? hk[e] : hpk[e][q];
-                    hqr = (q == n) ? hr[e] : hpr[e][q];
-                } else {
-                    hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
-                    hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
+            for (int q = 0; q < kMaxStages; ++q) {
+                if (q < a.nterms) {
+                    double hq, hqr;
+                    if constexpr (kHold) {
+                        hq = (q == n) ? hk[e] : hpk[e][q];
+                        hqr = (q == n) ? hr[e] : hpr[e][q];
+                    } else {
+                        hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
+                        hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
+                    }
+                    gK = gK + hq * a.coef[q];
+                    gR = gR + hqr * a.coef[q];
                 }
-                gK = gK + hq * a.coef[q];
-                gR = gR + hqr * a.coef[q];
+            }
+            if (a.last) {
+                double eK = g.x, eR = g.y;
+#pragma clang loop unroll(full)
+                for (int q = 0; q < kMaxStages; ++q) {
+                    if (q <= n) {
+                        double hq, hqr;
+                        if constexpr (kHold) {
+                            hq = (q == n) ? hk[e] : hpk[e][q];
+                            hqr = (q == n) ? hr[e] : hpr[e][q];
+                        } else {
+                            hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
+                            hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
+                        }
+                        eK = eK + hq * a.coefE[q];
+                        eR = eR + hqr * a.coefE[q];
+                    }
+                }
+                errj = fabs(gK - eK) + fabs(gR - eR);
             }
         }
         const double h0k = (n == 0) ? hk[e] : (kHold ? hpk[e][0] : W.hK[j]);
@@ -1154,25 +1196,7 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
         rr[0] += a.t_new * w * sR * gK;
         rr[1] += a.t_new * w * sK * gR;
         if (j == 0) { gK0 = gK; sKg0 = sK; }
-        if (a.last) {
-            double eK = g.x, eR = g.y;
-#pragma clang loop unroll(full)
-            for (int q = 0; q < kMaxStages; ++q) {
-                if (q <= n) {
-                    double hq, hqr;
-                    if constexpr (kHold) {
-                        hq = (q == n) ? hk[e] : hpk[e][q];
-                        hqr = (q == n) ? hr[e] : hpr[e][q];
-                    } else {
-                        hq = (q == n) ? hk[e] : W.hK[(size_t)q * N + j];
-                        hqr = (q == n) ? hr[e] : W.hR[(size_t)q * N + j];
-                    }
-                    eK = eK + hq * a.coefE[q];
-                    eR = eR + hqr * a.coefE[q];
-                }
-            }
-            err[0] += fabs(gK - eK) + fabs(gR - eR);
-        }
+        err[0] += errj;
     }
     block_sum<2, kStageThreads>(rr, red);
     if (a.last) block_sum<1, kStageThreads>(err, red);

@@ -161,7 +161,8 @@ int dmfe_get_state(dmfe_ctx* ctx, double* t1grid, double* QK, double* QR, double
 int dmfe_save_state(dmfe_ctx* ctx, const char* path);
 int dmfe_load_state(dmfe_ctx* ctx, const char* path, double* energy_out);
 
-/* Control scalars (config.delta_t, delta, delta_old, loop, rk->init). */
+/* Control scalars (config.delta_t, delta, delta_old, loop, rk->init: 1 RK54, 2 SSPRK104,
+ * 3..11 SERK2 with q = 2 (method - 2)). */
 int dmfe_set_control(dmfe_ctx* ctx, double delta_t, double delta, double delta_old, int64_t loop, int method);
 int dmfe_get_control(const dmfe_ctx* ctx, double* delta_t, double* delta, double* delta_old, int64_t* loop,
                      int* method);
@@ -207,6 +208,16 @@ int dmfe_loop_iteration(dmfe_ctx* ctx, double* t_out);
  * trip); results are bit-identical to repeated dmfe_loop_iteration calls.  Contexts
  * on dense grids and contexts with per-launch profiling on use the host loop. */
 int dmfe_run(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps);
+
+/* SERK2, the reference GPU loop's stabilized second-order-Chebyshev integrator
+ * (runge_kutta.cu:228-297,397-487, option use_serk2), off by default as on the CPU path:
+ * with it on, the every-maint_every maintenance moves SSPRK104 one SERK2 level up each
+ * time (method m >= 3 is SERK2 with q = 2 (m - 2): 2q stages of dt / (4q) in a Chebyshev
+ * recursion, the error against the embedded combination sum_k b_k g_k; step cap
+ * rmax[m - 1] / specRad) and a rollback steps SERK2 one level down.  Methods 3..11 can
+ * also be set directly (dmfe_set_control).  Not with sensitivities; dmfe_run uses the
+ * host-driven loop for SERK2. */
+int dmfe_set_serk2(dmfe_ctx* ctx, int on);
 
 /* The rollback of the reference's GPU main loop (simulation_runner.cpp:280-303,
  * rollbackState simulation_control.cpp:14-76), off by default as on the CPU path that

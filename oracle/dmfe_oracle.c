@@ -41,6 +41,8 @@ struct or_ctx {
     double delta_t_val, delta, delta_old;
     long loop;
     int rk_init;
+    int use_serk2; /* the GPU loop's SERK2 promotion at the maintenance (config.use_serk2) */
+    long maint_every; /* the maintenance cadence, 100000 in the reference (0: 100000) */
 };
 
 static double* dalloc(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
@@ -658,12 +660,156 @@ static void ss_init(void) {
     }
 }
 
-/* ---- runge_kutta.cpp:254-261 ---------------------------------------------------- */
+/* ---- SERK2 (runge_kutta.cu:228-297 coefficients, :397-487 stages) ---------------
+ * s = 2q stages of size alpha = 1/(4q): even stages an Euler step g += dt alpha h,
+ * odd stages the Chebyshev recursion g = 2 g - g_e + 2 dt alpha h (g_e: the value two
+ * stages back, i.e. after the previous odd stage); the stage values are
+ * T_i(x) T_2(x)^(j-1), x = 1 + alpha z, on y' = (z/dt) y.  b[0..s] combine them into
+ * the embedded solution that matches R(z) = T_s(w0 + w1 z) / T_s(w0) at z = 1..s+1,
+ * w0 = 1 + (19/20)/s^2, w1 = T_s(w0) / T_s'(w0); the error is |sum_k b_k g_k - g_s|. */
+static long double cheb_T(int n, long double x) {
+    long double a = 1.0L, b = x;
+    if (n == 0) return a;
+    for (int k = 1; k < n; ++k) { const long double c2 = 2.0L * x * b - a; a = b; b = c2; }
+    return b;
+}
+static long double cheb_U(int n, long double x) {
+    long double a = 1.0L, b = 2.0L * x;
+    if (n == 0) return a;
+    for (int k = 1; k < n; ++k) { const long double c2 = 2.0L * x * b - a; a = b; b = c2; }
+    return b;
+}
+void or_serk2_coeffs(int q, double* out) {
+    const int m = 2, s = m * q, n = s + 1;
+    const long double mu = 19.0L / 20.0L;
+    const long double a2 = (long double)q / ((long double)s * s);
+    const long double w0 = 1.0L + mu / ((long double)s * s);
+    const long double w1 = cheb_T(s, w0) / ((long double)s * cheb_U(s - 1, w0));
+    long double* A = (long double*)calloc((size_t)n * (n + 1), sizeof(long double)); /* [n][n | rhs] */
+    for (int r = 0; r < n; ++r) {
+        const long double z = (long double)(r + 1), x = 1.0L + a2 * z;
+        long double* row = A + (size_t)r * (n + 1);
+        row[0] = 1.0L;
+        long double tm = 1.0L; /* T_m(x)^(j-1) */
+        for (int j = 1; j <= q; ++j) {
+            for (int i = 1; i <= m; ++i) row[i + m * (j - 1)] = cheb_T(i, x) * tm;
+            tm *= cheb_T(m, x);
+        }
+        row[n] = cheb_T(s, w0 + w1 * z) / cheb_T(s, w0);
+    }
+    /* Gaussian elimination with partial pivoting */
+    for (int k = 0; k < n; ++k) {
+        int pr = k;
+        for (int r = k + 1; r < n; ++r)
+            if (fabsl(A[(size_t)r * (n + 1) + k]) > fabsl(A[(size_t)pr * (n + 1) + k])) pr = r;
+        if (pr != k)
+            for (int col = 0; col <= n; ++col) {
+                const long double t = A[(size_t)k * (n + 1) + col];
+                A[(size_t)k * (n + 1) + col] = A[(size_t)pr * (n + 1) + col];
+                A[(size_t)pr * (n + 1) + col] = t;
+            }
+        for (int r = k + 1; r < n; ++r) {
+            const long double f = A[(size_t)r * (n + 1) + k] / A[(size_t)k * (n + 1) + k];
+            for (int col = k; col <= n; ++col) A[(size_t)r * (n + 1) + col] -= f * A[(size_t)k * (n + 1) + col];
+        }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        long double v = A[(size_t)k * (n + 1) + n];
+        for (int col = k + 1; col < n; ++col) v -= A[(size_t)k * (n + 1) + col] * (long double)out[col];
+        out[k] = (double)(v / A[(size_t)k * (n + 1) + k]);
+    }
+    free(A);
+}
+
+static double serk2(or_ctx* c, int q) {
+    const size_t N = c->N;
+    const int S = 2 * q;
+    double* b = dalloc((size_t)S + 1);
+    or_serk2_coeffs(q, b);
+    const double alpha = 1.0 / (4 * q);
+    const double dt = c->delta_t_val;
+    double* gK = dalloc(N);
+    double* gR = dalloc(N);
+    double* eK = dalloc(N); /* g_e */
+    double* eR = dalloc(N);
+    double* fK = dalloc(N); /* sum_k b_k g_k */
+    double* fR = dalloc(N);
+    double* hK = dalloc(N);
+    double* hR = dalloc(N);
+    double* h0K = dalloc(N);
+    double* h0R = dalloc(N);
+    double* tmp1 = dalloc(N);
+    double* tmp2 = dalloc(N * N);
+    const double* q0K = c->QK + (c->T1 - 1) * N;
+    const double* q0R = c->QR + (c->T1 - 1) * N;
+    for (size_t i = 0; i < N; ++i) {
+        gK[i] = eK[i] = q0K[i];
+        gR[i] = eR[i] = q0R[i];
+        fK[i] = q0K[i] * b[0];
+        fR[i] = q0R[i] * b[0];
+    }
+    const double t0 = c->t1[c->T1 - 1];
+    double gt = t0, gte = t0;
+    const double gtf = t0 + dt;
+    double dr = 0.0;
+    for (int n = 0; n < S; ++n) {
+        if (n == 0) {
+            if (c->T1 == 1) interpolate_with(c, 0, NULL, NULL);
+        } else {
+            memcpy(tmp1, c->posB1old, N * sizeof(double));
+            memcpy(tmp2, c->posB2old, N * N * sizeof(double));
+            interpolate_with(c, 1, tmp1, tmp2);
+        }
+        or_rhs(c, hK, hR);
+        const double a = dt * alpha;
+        if (n == 0) {
+            memcpy(h0K, hK, N * sizeof(double));
+            memcpy(h0R, hR, N * sizeof(double));
+            dr = or_drstep2(c, q0K, q0R, hK, hR, t0);
+            for (size_t i = 0; i < N; ++i) { gK[i] = gK[i] + hK[i] * a; gR[i] = gR[i] + hR[i] * a; }
+            gt = t0 + dt * alpha * 1.0;
+        } else if (n % 2 == 0) {
+            for (size_t i = 0; i < N; ++i) { gK[i] = gK[i] + hK[i] * a; gR[i] = gR[i] + hR[i] * a; }
+            gt = gt + dt * alpha * 1.0;
+        } else {
+            const double a2 = 2 * dt * alpha;
+            for (size_t i = 0; i < N; ++i) {
+                gK[i] = 2 * gK[i] - eK[i] + hK[i] * a2;
+                gR[i] = 2 * gR[i] - eR[i] + hR[i] * a2;
+                eK[i] = gK[i];
+                eR[i] = gR[i];
+            }
+            gt = 2 * gt - gte + 2 * dt * alpha * 1.0;
+            gte = gt;
+        }
+        if (b[n + 1] != 0.0)
+            for (size_t i = 0; i < N; ++i) { fK[i] = fK[i] + gK[i] * b[n + 1]; fR[i] = fR[i] + gR[i] * b[n + 1]; }
+        if (n == 0) append_all(c, gK, gR, h0K, h0R, dr, gt);
+        else replace_all(c, gK, gR, h0K, h0R, dr, gt);
+    }
+    /* positions for the next step (the next step's first stage reuses them) */
+    memcpy(tmp1, c->posB1old, N * sizeof(double));
+    memcpy(tmp2, c->posB2old, N * N * sizeof(double));
+    interpolate_with(c, 1, tmp1, tmp2);
+    double err = 0.0;
+    for (size_t i = 0; i < N; ++i) err += fabs(fK[i] - gK[i]);
+    for (size_t i = 0; i < N; ++i) err += fabs(fR[i] - gR[i]);
+    err += fabs(gtf - gt);
+    free(b); free(gK); free(gR); free(eK); free(eR); free(fK); free(fR); free(hK); free(hR);
+    free(h0K); free(h0R); free(tmp1); free(tmp2);
+    return err;
+}
+
+/* ---- runge_kutta.cpp:254-261 (runge_kutta.cu:607-615 for SERK2) ------------------ */
 double or_update(or_ctx* c) {
     if (c->rk_init == 1) return rk_generic(c, 7, DP_A, DP_B, DP_B2, 0);
+    if (c->rk_init >= 3) return serk2(c, 2 * (c->rk_init - 2));
     ss_init();
     return rk_generic(c, 10, SS_A, SS_B, SS_B2, 1);
 }
+
+void or_set_serk2(or_ctx* c, int on) { c->use_serk2 = on; }
+void or_set_maint_every(or_ctx* c, long every) { c->maint_every = every; }
 
 /* ---- sparsify_utils.cpp:14-97 (CPU-only build) ------------------------------------ */
 void or_sparsify(or_ctx* c, double threshold) {
@@ -728,7 +874,7 @@ double or_loop_iteration(or_ctx* c) {
     c->delta_old = c->delta;
     c->delta = or_update(c);
     c->loop++;
-    if (c->loop % 100000 == 0) {
+    if (c->loop % (c->maint_every > 0 ? c->maint_every : 100000) == 0) {
         size_t prev = c->T1;
         int count = 0;
         while (count < 1) {
@@ -741,7 +887,11 @@ double or_loop_iteration(or_ctx* c) {
         /* last_rollback_loop stays at its initial -1000 on the CPU path */
         if (c->delta < c->delta_max / 2 && c->loop - (-1000) > 1000) {
             c->delta_t_val *= 0.5;
-            c->rk_init = 2;
+            /* CPU path: SSPRK104; GPU path with use_serk2: RK54 -> SSPRK104, else one
+             * SERK2 level up (init_SERK2(2 (init - 1)) sets init + 1), capped at the
+             * 11 rmax entries */
+            if (!c->use_serk2 || c->rk_init == 1) c->rk_init = 2;
+            else if (c->rk_init < 11) c->rk_init += 1;
         }
     }
     if (c->delta < c->delta_max && c->loop > 5 &&

@@ -55,6 +55,12 @@ void or_rhs(or_ctx* c, double* hK, double* hR);
 double or_rstep(or_ctx* c);
 double or_drstep2(or_ctx* c, const double* qK, const double* qR, const double* dqK, const double* dqR, double t);
 double or_update(or_ctx* c);
+/* SERK2 (the reference GPU loop's stabilized integrator): b[0..2q] for q, and the
+ * maintenance promotion to it (use_serk2, off = CPU path). */
+void or_serk2_coeffs(int q, double* out);
+void or_set_serk2(or_ctx* c, int on);
+/* the every-1e5-loops maintenance cadence (simulation_runner.cpp:153), lowered in tests */
+void or_set_maint_every(or_ctx* c, long every);
 double or_loop_iteration(or_ctx* c);
 double or_energy(or_ctx* c, double t);
 void or_sparsify(or_ctx* c, double threshold);

@@ -86,6 +86,9 @@ class Oracle(_Common):
         L.or_set_history.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 8
         L.or_bsearch.argtypes = [C.c_int64, _dp, C.c_int64, _dp, _dp]
         L.or_isearch.argtypes = [C.c_int64, _dp, C.c_int64, C.c_int64, _dp, _dp, _dp]
+        L.or_serk2_coeffs.argtypes = [C.c_int, _dp]
+        L.or_set_serk2.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_maint_every.argtypes = [C.c_void_p, C.c_long]
         for f in ("or_destroy", "or_init_quench", "or_update", "or_loop_iteration", "or_rstep", "or_rhs",
                   "or_get_fields", "or_get_history", "or_get_control", "or_set_pos_old", "or_get_pos_old"):
             if getattr(L, f).argtypes is None:
@@ -97,6 +100,17 @@ class Oracle(_Common):
         if self.h:
             self.L.or_destroy(self.h)
             self.h = None
+
+    def set_serk2(self, on):
+        self.L.or_set_serk2(self.h, 1 if on else 0)
+
+    def set_maint_every(self, every):
+        self.L.or_set_maint_every(self.h, every)
+
+    def serk2_coeffs(self, q):
+        out = np.zeros(2 * q + 1)
+        self.L.or_serk2_coeffs(q, _p(out))
+        return out
 
     def __del__(self):
         try:

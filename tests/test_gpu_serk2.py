@@ -1,0 +1,83 @@
+"""SERK2 on the GPU path against the oracle's SERK2 (tests/test_oracle_serk2.py pins the
+oracle): teacher-forced steps at two levels, and the main loop with the SERK2
+promotion at a lowered maintenance cadence (RK54 -> SSPRK104 -> SERK2(2) -> SERK2(4))."""
+import numpy as np
+import pytest
+
+import dynamite_amd as dm
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+_G = {}
+
+
+def grid():
+    if "g" not in _G:
+        _G["g"] = dm.generate_grid(64, 1e2, 5, dm.INTERP_POLY, 9, 0)
+    return _G["g"]
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("method", [3, 4])
+def test_serk2_steps_match_oracle(method):
+    g = grid()
+    o = Oracle(g)
+    o.init_quench()
+    for _ in range(150):
+        o.loop_iteration()
+    st = o.state()
+    ctx = dm.Context(g, dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.8))
+    ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
+    ctx.interpolate(dm.POS_BSEARCH)
+    o.interpolate(0)
+    dt = 2e-3
+    for k in range(6):
+        print(f"SERK2 method {method} step {k}", flush=True)
+        ctx.set_control(dt, 1.0, 0.0, 10, method)
+        o.set_control(dt, 1.0, 0.0, 10, method)
+        eg = ctx.rk_step()
+        eo = o.update()
+        sg, so = ctx.get_state(), o.state()
+        assert np.array_equal(sg["t1grid"], so["t1grid"])
+        for key in ("QK", "QR", "dQK", "dQR", "rvec", "drvec"):
+            assert rel(sg[key], so[key]) <= 1e-11, key
+        assert abs(eg - eo) <= 1e-6 * abs(eo) + 4 * 64 * 2.2e-16, (eg, eo)
+    ctx.close()
+    o.close()
+
+
+def test_main_loop_promotes_to_serk2_like_the_oracle():
+    g = grid()
+    p = dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.8)
+    p.maint_every = 60
+    p.delta_max = 1e-3  # loose: SSPRK104 is not switched back to RK54 between maintenances
+    ctx = dm.Context(g, p, rows_hint=64)
+    dm.set_serk2(ctx, True)
+    o = Oracle(g, delta_max=1e-3)
+    o.set_serk2(True)
+    o.set_maint_every(60)
+    ctx.init_quench()
+    o.init_quench()
+    c = ctx.get_control()
+    ctx.set_control(4e-5, c["delta"], c["delta_old"], c["loop"], c["method"])
+    o.set_control(4e-5, c["delta"], c["delta_old"], c["loop"], c["method"])
+    methods = []
+    for k in range(200):
+        tg = ctx.loop_iteration()
+        to = o.loop_iteration()
+        assert tg == to, k
+        mg, mo = ctx.get_control()["method"], o.control()["method"]
+        assert mg == mo, (k, mg, mo)
+        methods.append(mg)
+    # RK54 until loop 60, SSPRK104 until 120, SERK2(2) until 180, then SERK2(4)
+    assert methods[58] == 1 and methods[59] == 2 and methods[119] == 3 and methods[179] == 4, sorted(set(methods))
+    sg, so = ctx.get_state(), o.state()
+    assert np.array_equal(sg["t1grid"], so["t1grid"])
+    for key in ("QK", "QR"):
+        assert rel(sg[key], so[key]) <= 1e-9, key
+    ctx.close()
+    o.close()
