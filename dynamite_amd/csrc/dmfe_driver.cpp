@@ -28,6 +28,7 @@ struct Options {
     double tmax = 1e7, dt_min = 1e-5, delta_max = 1e-10;
     long long len = 512;
     bool save = true, log_response = false, resume = true;
+    bool serk2 = false, rollback = false;  // the reference GPU loop's behaviours (off: CPU path)
     std::string out = "Results", grid_dir;
     long long save_every = 0, maint_every = 100000;
 };
@@ -43,7 +44,9 @@ void usage() {
         "  --grid DIR        grid package directory (default Grid_data/<len>)\n"
         "  --save-every N    checkpoint every N loops (default: at the end only)\n"
         "  --maint-every N   sparsify/re-interpolate cadence (default 100000)\n"
-        "  --fresh           do not resume from an existing data.bin\n");
+        "  --fresh           do not resume from an existing data.bin\n"
+        "  -S true|false     SERK2 promotion at the maintenance (GPU-loop behaviour; default false)\n"
+        "  --rollback        roll back up to 10 steps on error spikes (GPU-loop behaviour)\n");
 }
 
 bool parse(int argc, char** argv, Options& o) {
@@ -74,7 +77,9 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "--save-every") o.save_every = (long long)std::atof(val("--save-every"));
         else if (a == "--maint-every") o.maint_every = (long long)std::atof(val("--maint-every"));
         else if (a == "--fresh") o.resume = false;
-        else if (a == "-D" || a == "-S" || a == "-g" || a == "-A" || a == "-w" || a == "-F" || a == "-I") (void)val(a.c_str());
+        else if (a == "-S" || a == "--serk2") o.serk2 = std::string(val("-S")) != "false";
+        else if (a == "--rollback") o.rollback = true;
+        else if (a == "-D" || a == "-g" || a == "-A" || a == "-w" || a == "-F" || a == "-I") (void)val(a.c_str());
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return false; }
     }
     if (o.grid_dir.empty()) o.grid_dir = "Grid_data/" + std::to_string(o.len);
@@ -174,6 +179,9 @@ int main(int argc, char** argv) {
     prm.maint_every = (int32_t)o.maint_every;
     dmfe_ctx* ctx = nullptr;
     if (!check(dmfe_ctx_create(&g, &prm, 4096, &ctx), "dmfe_ctx_create")) return 1;
+    if (!check(dmfe_set_serk2(ctx, o.serk2 ? 1 : 0), "dmfe_set_serk2") ||
+        !check(dmfe_set_rollback(ctx, o.rollback ? 1 : 0), "dmfe_set_rollback"))
+        return 1;
     const std::string dir = param_dir(o);
     const std::string bin = dir + "/data.bin";
     bool resumed = false;
