@@ -91,7 +91,7 @@ struct dmfe_batch {
     int2* d_smap = nullptr;
     int4* d_rmap = nullptr;
     int nb_stage = 0, nb_rhs = 0, ept = 1, G = 8, maxm = 1, log_response = 0;
-    int tanK = 0;  // device loop of a context with tangents: tangent RHS and finish launches
+    int tanK = 0;  // device loop of a context with tangents (launch shapes depend on it)
     size_t lds_rhs = 0;
     double* d_obs = nullptr;
     int obs_cap = 0;
@@ -179,6 +179,7 @@ int enqueue_step(dmfe_batch* b, int S) {
         for (int k = 0; k < S; ++k) {
             launch_loop_stage(d, k, b->stream);
             launch_loop_rhs(d, k, b->log_response, b->G, b->lds_rhs, b->stream);
+            // before the primal finish of stage k (next stage launch), as in the host loop
             if (d.X.K) launch_loop_tan_finish(d, k, b->stream);
         }
         launch_loop_stage(d, S, b->stream);
@@ -190,13 +191,7 @@ int enqueue_step(dmfe_batch* b, int S) {
     launch_batch_begin(b->d_desc, n, b->stream);
     for (int k = 0; k < S; ++k) {
         launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);
-        if (b->tanK) {
-            // before the primal finish of stage k (next stage launch), as in the host loop
-            launch_batch_rhs_tan(b->d_desc, b->d_rmap, b->nb_rhs, k, b->G, b->tanK, b->lds_rhs, b->stream);
-            launch_batch_tan_finish(b->d_desc, 0, k, b->grids[0].N, b->tanK, b->stream);
-        } else {
-            launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->maxm, b->lds_rhs, b->stream);
-        }
+        launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->maxm, b->lds_rhs, b->stream);
     }
     launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, S, b->ept, b->stream);
     launch_batch_end(b->d_desc, n, b->stream);

@@ -1560,21 +1560,7 @@ __global__ __launch_bounds__(kRowThreads) void loop_rhs_kernel(InstDesc d, int n
     rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red);
 }
 
-// the same RHS launch with the tangent quadratures of the instance's K directions
-template <int G, int K>
-__global__ __launch_bounds__(kRowThreads) void batch_rhs_tan_kernel(const InstDesc* __restrict__ D,
-                                                                    const int4* __restrict__ map, int n) {
-    extern __shared__ double2 dyn[];
-    __shared__ double red[kSums * (kRowThreads / kWave)];
-    const int4 mp = map[xcd_remap(blockIdx.x, gridDim.x)];
-    if (mp.x < 0) return;
-    const InstDesc& d = D[mp.x];
-    const Ctl& c = *d.ctl;
-    if (!c.active || n >= c.S) return;
-    const RhsArgs a = step_rhs_args(c, d, n, 0);
-    rhs_rows_body<false, G, K>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red, &d.X);
-}
-
+// the device loop's RHS with the tangent quadratures of the instance's K directions
 template <int G, int K>
 __global__ __launch_bounds__(kRowThreads) void loop_rhs_tan_kernel(InstDesc d, int n) {
     extern __shared__ double2 dyn[];
@@ -1873,18 +1859,8 @@ __global__ __launch_bounds__(kStageThreads) void tan_finish_kernel(Tables T, His
     tan_finish_body<EPT, K>(T, H, W, M, a, X, red);
 }
 
-// device-side step control with tangents: the tangent finish of stage n of instance
-// `inst` from the stage arguments its begin kernel wrote (the RHS buffer of stage n)
-template <int EPT, int K>
-__global__ __launch_bounds__(kStageThreads) void batch_tan_finish_kernel(const InstDesc* __restrict__ D, int inst,
-                                                                         int n) {
-    __shared__ double red[6 * K * (kStageThreads / kWave)];
-    const InstDesc& d = D[inst];
-    const Ctl& c = *d.ctl;
-    if (!c.active || n >= c.S) return;
-    tan_finish_body<EPT, K>(d.T, d.H, d.W, d.M, c.fin[n], d.X, red);
-}
-
+// device loop of one context with tangents: the tangent finish of stage n from the stage
+// arguments its begin kernel wrote (the RHS buffer of stage n)
 template <int EPT, int K>
 __global__ __launch_bounds__(kStageThreads) void loop_tan_finish_kernel(InstDesc d, int n) {
     __shared__ double red[6 * K * (kStageThreads / kWave)];
@@ -2096,36 +2072,6 @@ void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_
     hipLaunchKernelGGL((batch_rhs_kernel<LOG, GG, NT>), dim3(nblocks), dim3(kRowThreads), lds, s, D, map, n)
     DMFE_SHAPES(log_response, G, b2_taps_per_pass(G, maxm), DMFE_BRHS)
 #undef DMFE_BRHS
-}
-
-void launch_batch_rhs_tan(InstDesc* D, const int4* map, int nblocks, int n, int G, int K, size_t lds, hipStream_t s) {
-#define DMFE_BRT(GG, KK) \
-    hipLaunchKernelGGL((batch_rhs_tan_kernel<GG, KK>), dim3(nblocks), dim3(kRowThreads), lds, s, D, map, n)
-#define DMFE_BRTG(KK)                       \
-    switch (G) {                            \
-        case 4: DMFE_BRT(4, KK); break;     \
-        case 8: DMFE_BRT(8, KK); break;     \
-        case 16: DMFE_BRT(16, KK); break;   \
-        default: DMFE_BRT(64, KK); break;   \
-    }
-    if (K == 1) DMFE_BRTG(1)
-    else DMFE_BRTG(2)
-#undef DMFE_BRTG
-#undef DMFE_BRT
-}
-
-void launch_batch_tan_finish(InstDesc* D, int inst, int n, int N, int K, hipStream_t s) {
-    const int ept = (N + kStageThreads - 1) / kStageThreads;
-#define DMFE_BTF(E, KK) hipLaunchKernelGGL((batch_tan_finish_kernel<E, KK>), dim3(1), dim3(kStageThreads), 0, s, D, inst, n)
-#define DMFE_BTFE(KK)                       \
-    if (ept <= 1) DMFE_BTF(1, KK);          \
-    else if (ept <= 2) DMFE_BTF(2, KK);     \
-    else if (ept <= 4) DMFE_BTF(4, KK);     \
-    else DMFE_BTF(8, KK);
-    if (K == 1) { DMFE_BTFE(1) }
-    else { DMFE_BTFE(2) }
-#undef DMFE_BTFE
-#undef DMFE_BTF
 }
 
 // ---- device loop of one context (instance by value) ----
