@@ -162,3 +162,27 @@ def test_run_with_direct_launches_matches_graph_replays():
     assert np.array_equal(out[0][1], out[1][1])
     for k in ("t1grid", "QK", "QR", "dQK", "dQR", "rvec"):
         assert np.array_equal(out[0][0][k], out[1][0][k]), k
+
+
+@pytest.mark.parametrize("kind", ["log_response", "rational"])
+def test_run_matches_host_loop_other_forms(kind):
+    """The device loop with the log-response interpolation and on a rational-stencil grid
+    (wider stencil, another RHS instantiation)."""
+    if kind == "rational":
+        g = dm.generate_grid(64, 1e2, 5, dm.INTERP_RATIONAL, 9, 13)
+        log_resp = 0
+    else:
+        g = grid(64)
+        log_resp = 1
+    print(f"device loop, {kind}: m={g.m}", flush=True)
+    p = dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.8, log_response=log_resp)
+    dev = dm.Context(g, p, rows_hint=64)
+    host = dm.Context(g, p, rows_hint=64)
+    for c in (dev, host):
+        c.init_quench()
+    n, corr = dev.run(float("inf"), 150, corr_cap=150)
+    ref = host_run(host, n)
+    assert n == 150 and np.array_equal(corr, ref)
+    same_state(dev, host)
+    dev.close()
+    host.close()
