@@ -224,6 +224,7 @@ struct Ctl {
     double gt[kMaxStages + 1];
     double tprev;       // t1grid[T1s - 2]
     double gt_err;      // |gt_S - gte|
+    double rho_step;    // rho of the row the step writes (set with the last stage's arguments)
     // launch arguments of this step, written by the begin kernel
     FinishArgs fin[kMaxStages];      // finish of stage m
     SearchArgs srch[kMaxStages + 1]; // searches of stage n (mode < 0: none)

@@ -1324,12 +1324,23 @@ __constant__ double cA[2][kMaxStages][kMaxStages];
 __constant__ double cB[2][kMaxStages];
 __constant__ double cB2[2][kMaxStages];
 
+// The per-step scalars that the begin kernel's argument builders read: thread 0 writes
+// them to LDS as well as to the instance's Ctl, so the builders need no global round trip.
+struct StepHead {
+    int T1s, S, fc0, method;
+    double dt, tprev, gt_err;
+    double gt[kMaxStages + 1];
+};
+
 #pragma clang fp contract(off)
-__device__ inline int searches_before(const Ctl& c, int k) {  // searches in stages 0..k-1 of this step
+// C is Ctl or StepHead
+template <class C>
+__device__ inline int searches_before(const C& c, int k) {  // searches in stages 0..k-1 of this step
     return k <= 0 ? 0 : (c.T1s == 1 ? 1 : 0) + (k - 1);
 }
 
-__device__ inline bool make_search_dev(const Ctl& c, const InstDesc& d, int n, SearchArgs& a) {
+template <class C>
+__device__ inline bool make_search_dev(const C& c, const InstDesc& d, int n, SearchArgs& a) {
     const int S = c.S;
     const bool ss = c.method != DMFE_RK54;
     a = SearchArgs{};
@@ -1361,7 +1372,8 @@ __device__ inline bool make_search_dev(const Ctl& c, const InstDesc& d, int n, S
     return true;
 }
 
-__device__ inline void make_finish_dev(const Ctl& c, const InstDesc& d, int m, FinishArgs& f) {
+template <class C>
+__device__ inline void make_finish_dev(const C& c, const InstDesc& d, int m, FinishArgs& f) {
     const int S = c.S;
     const int t = c.method == DMFE_RK54 ? 0 : 1;
     const bool last = (m == S - 1);
@@ -1393,88 +1405,122 @@ __device__ inline void make_finish_dev(const Ctl& c, const InstDesc& d, int m, F
 // One block per instance: lane 0 runs the scalar control, then lanes 0..S build the
 // search arguments of stages 0..S and lanes 16..16+S-1 the finish arguments.
 constexpr int kBeginThreads = 32;
-__device__ __forceinline__ void begin_body(const InstDesc& d, int* go_lds) {
+__device__ __forceinline__ void begin_body(const InstDesc& d, int* go_lds, StepHead* sh) {
     Ctl& c = *d.ctl;
     int& go = *go_lds;
     if (threadIdx.x == 0) {
+        // all control fields are loaded before the first test and the first store (one
+        // memory round trip), then the two time-grid entries together (the second)
+        const int status = c.status, T1 = c.T1, cap = c.cap, method = c.method, fcur = c.fcur;
+        const long long loop = c.loop, max_loop = c.max_loop;
+        const double dt = c.dt, dt_min = c.dt_min, t_prop = c.t_prop, tmax = c.tmax, delta = c.delta;
         go = 0;
         c.active = 0;
         const double eps = 2.220446049250313e-16;
-        const double tol = eps * fmax(fabs(c.dt), fabs(c.dt_min));
-        if (c.status != kRunning) {
-        } else if (!(c.t_prop < c.tmax && c.loop < c.max_loop && c.dt >= c.dt_min - tol)) {
+        const double tol = eps * fmax(fabs(dt), fabs(dt_min));
+        if (status != kRunning) {
+        } else if (!(t_prop < tmax && loop < max_loop && dt >= dt_min - tol)) {
             c.status = kDone;
-        } else if (c.T1 + 1 > c.cap) {
+        } else if (T1 + 1 > cap) {
             c.status = kNeedRows;
         } else {
             const double* t1 = d.H.t1;
-            c.T1s = c.T1;
-            c.gt[0] = t1[c.T1 - 1];
-            c.tprev = c.T1 >= 2 ? t1[c.T1 - 2] : 0.0;
-            c.t_prop = c.gt[0] + c.dt;
-            c.delta_old = c.delta;
-            const int t = c.method == DMFE_RK54 ? 0 : 1;
+            const double g0 = t1[T1 - 1];
+            const double tprev = T1 >= 2 ? t1[T1 - 2] : 0.0;
+            const int t = method == DMFE_RK54 ? 0 : 1;
             const int S = t == 0 ? 7 : 10;
-            c.S = S;
-            const double dt = c.dt;
-            c.gt[1] = c.gt[0] + dt * cA[t][1][0] * 1.0;
+            StepHead& h = *sh;
+            h.T1s = T1;
+            h.S = S;
+            h.fc0 = fcur;
+            h.method = method;
+            h.dt = dt;
+            h.tprev = tprev;
+            h.gt[0] = g0;
+            h.gt[1] = g0 + dt * cA[t][1][0] * 1.0;
             for (int m = 1; m < S; ++m) {
                 const bool last = (m == S - 1);
-                double g = c.gt[0];
+                double g = g0;
                 const int nt = last ? S : m + 1;
                 for (int j = 0; j < nt; ++j) g += dt * (last ? cB[t][j] : cA[t][m + 1][j]) * 1.0;
-                c.gt[m + 1] = g;
+                h.gt[m + 1] = g;
             }
-            double gte = c.gt[0];
+            double gte = g0;
             for (int j = 0; j < S; ++j) gte += dt * cB2[t][j] * 1.0;
-            c.gt_err = fabs(c.gt[S] - gte);
-            c.fc0 = c.fcur;
+            h.gt_err = fabs(h.gt[S] - gte);
+            c.T1s = T1;
+            c.S = S;
+            c.fc0 = fcur;
+            c.tprev = tprev;
+            c.gt_err = h.gt_err;
+            for (int m = 0; m <= S; ++m) c.gt[m] = h.gt[m];
+            c.t_prop = g0 + dt;
+            c.delta_old = delta;
+            // rho of the appended row, as the last stage's finish writes it (make_finish_dev)
+            c.rho_step = (T1 > 1) ? (h.gt[S] - g0) / (g0 - tprev) : 0.0;
             go = 1;
         }
     }
     __syncthreads();
     if (!go) return;
+    const StepHead& h = *sh;
     const int k = threadIdx.x;
-    if (k <= c.S) make_search_dev(c, d, k, c.srch[k]);
-    else if (k >= 16 && k - 16 < c.S) make_finish_dev(c, d, k - 16, c.fin[k - 16]);
+    if (k <= h.S) make_search_dev(h, d, k, c.srch[k]);
+    else if (k >= 16 && k - 16 < h.S) make_finish_dev(h, d, k - 16, c.fin[k - 16]);
     if (k == 0) c.active = 1;
 }
 
 __global__ __launch_bounds__(kBeginThreads) void batch_begin_kernel(InstDesc* D, int n) {
     __shared__ int go;
-    if ((int)blockIdx.x < n) begin_body(D[blockIdx.x], &go);
+    __shared__ StepHead sh;
+    if ((int)blockIdx.x < n) begin_body(D[blockIdx.x], &go, &sh);
 }
 
 // device loop of one context: the instance comes by value (no descriptor loads)
 __global__ __launch_bounds__(kBeginThreads) void loop_begin_kernel(InstDesc d) {
     __shared__ int go;
-    begin_body(d, &go);
+    __shared__ StepHead sh;
+    begin_body(d, &go, &sh);
 }
 
 // end: delta, loop, maintenance cadence and the adaptive dt rule (simulation_runner.cpp:144-278)
 __device__ __forceinline__ void end_body(const InstDesc& d) {
     Ctl& c = *d.ctl;
-    if (!c.active) return;  // stays set until the next begin: the observables launch reads it
-    c.delta = d.W.scal[1];
-    c.loop += 1;
-    c.T1 = c.T1s + 1;
-    c.fcur = (c.fc0 + searches_before(c, c.S) + 1) & 1;
-    if (c.maint_every > 0 && c.loop % c.maint_every == 0) {
+    // every operand is loaded before the first test and the first store (one memory
+    // round trip); rho of the new last row is the value its finish wrote (Ctl::rho_step)
+    const int active = c.active, T1s = c.T1s, fc0 = c.fc0, S = c.S, method = c.method;
+    const long long loop = c.loop + 1, maint_every = c.maint_every;
+    const double delta = d.W.scal[1], delta_old = c.delta_old, dmax = c.delta_max, specRad = c.specRad;
+    const double dt_min = c.dt_min, rho_last = c.rho_step;
+    double dt = c.dt;
+    double rmax[11];
+#pragma unroll
+    for (int q = 0; q < 11; ++q) rmax[q] = c.rmax[q];
+    if (!active) return;  // stays set until the next begin: the observables launch reads it
+    c.delta = delta;
+    c.loop = loop;
+    c.T1 = T1s + 1;
+    StepHead h{};
+    h.T1s = T1s;
+    c.fcur = (fc0 + searches_before(h, S) + 1) & 1;
+    if (maint_every > 0 && loop % maint_every == 0) {
         c.status = kNeedMaint;  // sparsify, interpolate and the dt rule run on the host
         return;
     }
-    const double dmax = c.delta_max;
-    const double rho_last = d.H.rho[c.T1 - 1];
-    if (c.delta < dmax && c.loop > 5 && (c.delta < 1.1 * c.delta_old || c.delta_old < dmax / 1000) &&
-        c.rmax[c.method - 1] / c.specRad > c.dt && rho_last == 1.0) {
-        c.dt *= 1.01;
-    } else if (c.delta > 2 * dmax && c.dt > c.dt_min) {
-        c.dt *= 0.9;
+    double rm = rmax[0];  // rmax[method - 1] without dynamic register indexing
+#pragma unroll
+    for (int q = 1; q < 11; ++q) rm = (method - 1 == q) ? rmax[q] : rm;
+    if (delta < dmax && loop > 5 && (delta < 1.1 * delta_old || delta_old < dmax / 1000) && rm / specRad > dt &&
+        rho_last == 1.0) {
+        dt *= 1.01;
+    } else if (delta > 2 * dmax && dt > dt_min) {
+        dt *= 0.9;
     }
-    if (c.method == DMFE_SSPRK104 && c.delta > dmax && c.rmax[0] / c.specRad > c.dt) {
+    if (method == DMFE_SSPRK104 && delta > dmax && rmax[0] / specRad > dt) {
         c.method = DMFE_RK54;
-        c.dt *= 0.5;
+        dt *= 0.5;
     }
+    c.dt = dt;
 }
 
 __global__ void batch_end_kernel(InstDesc* D, int n) {
@@ -1492,13 +1538,24 @@ __global__ void loop_end_kernel(InstDesc d) {
 template <int EPT, bool PRE>
 __device__ __forceinline__ void stage_role_body(const InstDesc& d, int role, int n, double* lds, double* red) {
     const Ctl& c = *d.ctl;
-    if (!c.active || n > c.S) return;
+    // the control word and this block's arguments are loaded together (one memory round
+    // trip) before they are tested.  The device loop (PRE, about two blocks per CU) also
+    // keeps its finish arguments in registers across the block's global stores, which
+    // could otherwise alias them; the many-instance batch keeps occupancy instead.
     if (role == -2) {
-        if (n >= 1) finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
+        if (n < 1) return;
+        if constexpr (PRE) {
+            const int active = c.active, S = c.S;
+            const FinishArgs fa = c.fin[n - 1];
+            if (active && n <= S) finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, fa, red);
+        } else if (c.active && n <= c.S) {
+            finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
+        }
         return;
     }
-    const SearchArgs& a = c.srch[n];
-    if (a.mode < 0) return;
+    const int active = c.active, S = c.S;
+    const SearchArgs a = c.srch[n];
+    if (!active || n > S || a.mode < 0) return;
     if (role == -1) search_theta_row<EPT>(d.T, d.W, a, lds, red);
     else if (!(a.mode == kSearchSame && !a.saveB2)) search_phi2_row<EPT>(d.T, d.W, a, role, lds);
 }
@@ -1543,8 +1600,9 @@ __global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* 
     if (mp.x < 0) return;
     const InstDesc& d = D[mp.x];
     const Ctl& c = *d.ctl;
-    if (!c.active || n >= c.S) return;
-    const RhsArgs a = step_rhs_args(c, d, n, LOGR);
+    const int active = c.active, S = c.S;
+    const RhsArgs a = step_rhs_args(c, d, n, LOGR);  // loaded with the control word
+    if (!active || n >= S) return;
     rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red);
 }
 
@@ -1554,8 +1612,9 @@ __global__ __launch_bounds__(kRowThreads) void loop_rhs_kernel(InstDesc d, int n
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
     const Ctl& c = *d.ctl;
-    if (!c.active || n >= c.S) return;
-    const RhsArgs a = step_rhs_args(c, d, n, LOGR);
+    const int active = c.active, S = c.S;
+    const RhsArgs a = step_rhs_args(c, d, n, LOGR);  // loaded with the control word
+    if (!active || n >= S) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red);
 }
@@ -1566,8 +1625,9 @@ __global__ __launch_bounds__(kRowThreads) void loop_rhs_tan_kernel(InstDesc d, i
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
     const Ctl& c = *d.ctl;
-    if (!c.active || n >= c.S) return;
-    const RhsArgs a = step_rhs_args(c, d, n, 0);
+    const int active = c.active, S = c.S;
+    const RhsArgs a = step_rhs_args(c, d, n, 0);  // loaded with the control word
+    if (!active || n >= S) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     rhs_rows_body<false, G, K>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red, &d.X);
 }
