@@ -1029,6 +1029,8 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
     constexpr int kPre = kHold ? kMaxStages : 1;
     double2 g0v[EPT];
     double hpk[EPT][kPre], hpr[EPT][kPre];
+    // drstep2 of this step (stage 0 wrote it), read before this block's stores
+    const double scal0 = (a.mode != 0 && n != 0) ? W.scal[0] : 0.0;
     if (a.mode != 0) {
         const double2* g0 = H.V + (size_t)a.g0row * N;
 #pragma unroll
@@ -1115,7 +1117,7 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
     }
     double dr;
     if (n == 0) dr = red[0];  // LDS and global reads kept apart (no generic-pointer load)
-    else dr = W.scal[0];
+    else dr = scal0;
     // stage combination g_{n+1} = g_0 + sum_j h_j * (dt a_{n+1,j}) and the row write
     double2* wv = H.V + (size_t)a.wrow * N;
     double2* wd = H.D + (size_t)a.wrow * N;
@@ -1752,6 +1754,10 @@ __device__ __forceinline__ void tan_finish_body(const Tables& T, const History& 
     const int* Fl = W.Floor + (size_t)a.fbuf * N;
     const double* Df = W.diff + (size_t)a.fbuf * N;
     const double qK0 = last[0].x;
+    // tangent drstep2 of this step (stage 0 wrote it), read before this block's stores
+    double tdr0[K];
+#pragma unroll
+    for (int h = 0; h < K; ++h) tdr0[h] = (n != 0) ? X.scal[h * 8 + 0] : 0.0;
     const double s1 = fl1(M, qK0) / M.T0, s2 = -rint_at(H, Fl, Df, N - 1, a.lr + 1, live), s3 = qK0 / M.T0,
                  s4 = 2 * M.Gamma;
     double ds1[K], ds2[K], ds3[K], ds4[K];
@@ -1860,7 +1866,7 @@ __device__ __forceinline__ void tan_finish_body(const Tables& T, const History& 
         }
     } else {
 #pragma unroll
-        for (int h = 0; h < K; ++h) tdr[h] = X.scal[h * 8 + 0];
+        for (int h = 0; h < K; ++h) tdr[h] = tdr0[h];
     }
     // tangent stage combination, row write and rstep (time_steps.cpp:57-92,140-157)
     const double2* g0 = H.V + (size_t)a.g0row * N;
