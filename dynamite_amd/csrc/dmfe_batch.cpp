@@ -178,9 +178,9 @@ int enqueue_step(dmfe_batch* b, int S) {
         launch_loop_begin(d, b->stream);
         for (int k = 0; k < S; ++k) {
             launch_loop_stage(d, k, b->stream);
+            // with tangents the next stage launch runs their finish of stage k before the
+            // primal finish, as the host loop does
             launch_loop_rhs(d, k, b->log_response, b->G, b->lds_rhs, b->stream);
-            // before the primal finish of stage k (next stage launch), as in the host loop
-            if (d.X.K) launch_loop_tan_finish(d, k, b->stream);
         }
         launch_loop_stage(d, S, b->stream);
         launch_loop_end(d, b->stream);

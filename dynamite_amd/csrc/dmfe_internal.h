@@ -287,8 +287,7 @@ void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t 
 // the step launches of one context's device loop, the instance passed by value
 void launch_loop_begin(const InstDesc& d, hipStream_t s);
 void launch_loop_end(const InstDesc& d, hipStream_t s);
-void launch_loop_stage(const InstDesc& d, int n, hipStream_t s);
+void launch_loop_stage(const InstDesc& d, int n, hipStream_t s);  // with tangents: also their finish
 void launch_loop_rhs(const InstDesc& d, int n, int log_response, int G, size_t lds, hipStream_t s);
-void launch_loop_tan_finish(const InstDesc& d, int n, hipStream_t s);
 
 }  // namespace dmfe

@@ -1925,14 +1925,27 @@ __global__ __launch_bounds__(kStageThreads) void tan_finish_kernel(Tables T, His
     tan_finish_body<EPT, K>(T, H, W, M, a, X, red);
 }
 
-// device loop of one context with tangents: the tangent finish of stage n from the stage
-// arguments its begin kernel wrote (the RHS buffer of stage n)
+// Stage launch n of the device loop of a context with tangents: block 0 runs the tangent
+// finish of stage n-1 and then, after a barrier, the primal finish (which overwrites the
+// RHS's last row that the tangent finish reads), as the host loop's two launches do; the
+// other blocks are the searches of stage n.
 template <int EPT, int K>
-__global__ __launch_bounds__(kStageThreads) void loop_tan_finish_kernel(InstDesc d, int n) {
-    __shared__ double red[6 * K * (kStageThreads / kWave)];
-    const Ctl& c = *d.ctl;
-    if (!c.active || n >= c.S) return;
-    tan_finish_body<EPT, K>(d.T, d.H, d.W, d.M, c.fin[n], d.X, red);
+__global__ __launch_bounds__(kStageThreads) void loop_stage_tan_kernel(InstDesc d, int n) {
+    __shared__ double lds[EPT * kStageThreads];
+    __shared__ double red[(6 * K > 8 ? 6 * K : 8) * (kStageThreads / kWave)];
+    const int role = (int)blockIdx.x - 2;
+    if (role == -2) {
+        if (n < 1) return;
+        const Ctl& c = *d.ctl;
+        const int active = c.active, S = c.S;
+        const FinishArgs fa = c.fin[n - 1];
+        if (!active || n > S) return;
+        tan_finish_body<EPT, K>(d.T, d.H, d.W, d.M, c.fin[n - 1], d.X, red);  // (indexes its coefficients)
+        __syncthreads();
+        finish_block<EPT, true>(d.T, d.H, d.W, d.M, fa, red);
+        return;
+    }
+    stage_role_body<EPT, true>(d, role, n, lds, red);
 }
 
 // Tangent of the energy at time t on history row `row` (simulation_runner.cpp:308-318)
@@ -2152,6 +2165,19 @@ void launch_loop_end(const InstDesc& d, hipStream_t s) {
 void launch_loop_stage(const InstDesc& d, int n, hipStream_t s) {
     const int N = d.T.N, ept = (N + kStageThreads - 1) / kStageThreads;
     const dim3 grid((unsigned)(N + 2));
+    if (d.X.K) {  // with the tangent finish of stage n-1 in block 0
+#define DMFE_LST(E, KK) hipLaunchKernelGGL((loop_stage_tan_kernel<E, KK>), grid, dim3(kStageThreads), 0, s, d, n)
+#define DMFE_LSTE(KK)                       \
+    if (ept <= 1) DMFE_LST(1, KK);          \
+    else if (ept <= 2) DMFE_LST(2, KK);     \
+    else if (ept <= 4) DMFE_LST(4, KK);     \
+    else DMFE_LST(8, KK);
+        if (d.X.K == 1) { DMFE_LSTE(1) }
+        else { DMFE_LSTE(2) }
+#undef DMFE_LSTE
+#undef DMFE_LST
+        return;
+    }
     if (ept <= 1) hipLaunchKernelGGL(loop_stage_kernel<1>, grid, dim3(kStageThreads), 0, s, d, n);
     else if (ept <= 2) hipLaunchKernelGGL(loop_stage_kernel<2>, grid, dim3(kStageThreads), 0, s, d, n);
     else if (ept <= 4) hipLaunchKernelGGL(loop_stage_kernel<4>, grid, dim3(kStageThreads), 0, s, d, n);
@@ -2178,20 +2204,6 @@ void launch_loop_rhs(const InstDesc& d, int n, int log_response, int G, size_t l
 #define DMFE_LR(LOG, GG, NT) hipLaunchKernelGGL((loop_rhs_kernel<LOG, GG, NT>), grid, dim3(kRowThreads), lds, s, d, n)
     DMFE_SHAPES(log_response, G, b2_taps_per_pass(G, d.T.m), DMFE_LR)
 #undef DMFE_LR
-}
-
-void launch_loop_tan_finish(const InstDesc& d, int n, hipStream_t s) {
-    const int ept = (d.T.N + kStageThreads - 1) / kStageThreads;
-#define DMFE_LTF(E, KK) hipLaunchKernelGGL((loop_tan_finish_kernel<E, KK>), dim3(1), dim3(kStageThreads), 0, s, d, n)
-#define DMFE_LTFE(KK)                       \
-    if (ept <= 1) DMFE_LTF(1, KK);          \
-    else if (ept <= 2) DMFE_LTF(2, KK);     \
-    else if (ept <= 4) DMFE_LTF(4, KK);     \
-    else DMFE_LTF(8, KK);
-    if (d.X.K == 1) { DMFE_LTFE(1) }
-    else { DMFE_LTFE(2) }
-#undef DMFE_LTFE
-#undef DMFE_LTF
 }
 
 void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t s) {
