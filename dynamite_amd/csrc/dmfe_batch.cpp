@@ -99,6 +99,7 @@ struct dmfe_batch {
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // 7- and 10-stage step sequences
     int graph_steps[2] = {0, 0};
     bool graph_obs[2] = {false, false};
+    bool graph_s0[2] = {false, false};  // the graph holds the stage-0 search launch
     int64_t steps_run = 0;
     bool borrowed = false;  // device_loop_run: tables, instance buffers and stream belong to a context
     InstDesc desc0{};       // device loop: instance 0's descriptor, passed by value to its kernels
@@ -170,14 +171,25 @@ int interpolate_inst(dmfe_batch* b, int i) {
     return DMFE_OK;
 }
 
+// The stage-0 launch holds only the search of the first step (T1 = 1: the bsearch of
+// initialization); later steps reuse the previous step's last interpolation and there is
+// no finish before stage 0, so the launch is left out once every running instance has
+// two or more rows (T1 only grows between the host's looks at the instances).
+bool needs_stage0(const dmfe_batch* b) {
+    for (const Ctl& c : b->ctl)
+        if (c.status == kRunning && c.T1 <= 1) return true;
+    return false;
+}
+
 // One batch step: begin | (stage n, rhs n) for n < S | stage S | end [| observables]
 int enqueue_step(dmfe_batch* b, int S) {
     const int n = n_inst(b);
+    const bool s0 = needs_stage0(b);
     if (b->borrowed) {  // one context: descriptor by value, no launch maps
         const InstDesc& d = b->desc0;
         launch_loop_begin(d, b->stream);
         for (int k = 0; k < S; ++k) {
-            launch_loop_stage(d, k, b->stream);
+            if (k > 0 || s0) launch_loop_stage(d, k, b->stream);
             // with tangents the next stage launch runs their finish of stage k before the
             // primal finish, as the host loop does
             launch_loop_rhs(d, k, b->log_response, b->G, b->lds_rhs, b->stream);
@@ -190,7 +202,7 @@ int enqueue_step(dmfe_batch* b, int S) {
     }
     launch_batch_begin(b->d_desc, n, b->stream);
     for (int k = 0; k < S; ++k) {
-        launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);
+        if (k > 0 || s0) launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);
         launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->maxm, b->lds_rhs, b->stream);
     }
     launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, S, b->ept, b->stream);
@@ -208,7 +220,10 @@ int max_stages(const dmfe_batch* b) {
 }
 
 int build_graph(dmfe_batch* b, int which, int S, int steps) {
-    if (b->graph[which] && b->graph_steps[which] == steps && b->graph_obs[which] == b->obs_on) return DMFE_OK;
+    const bool s0 = needs_stage0(b);
+    if (b->graph[which] && b->graph_steps[which] == steps && b->graph_obs[which] == b->obs_on &&
+        b->graph_s0[which] == s0)
+        return DMFE_OK;
     if (b->graph[which]) (void)hipGraphExecDestroy(b->graph[which]);
     b->graph[which] = nullptr;
     hipGraph_t g = nullptr;
@@ -225,6 +240,7 @@ int build_graph(dmfe_batch* b, int which, int S, int steps) {
     if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
     b->graph_steps[which] = steps;
     b->graph_obs[which] = b->obs_on;
+    b->graph_s0[which] = s0;
     return DMFE_OK;
 }
 
