@@ -6,7 +6,8 @@ The quench is first integrated from its initial condition for --prep steps (a re
 history of T1 ~ prep rows, like the reference's [BENCH] harness runs on a live
 state); then `steps` main-loop iterations are timed.  One step = one adaptive RK
 step (RK54: 7 RHS evaluations with their interpolations, plus the dt rule)
-through the C ABI (dmfe_loop_iteration); all inputs resident in HBM.
+through the C ABI (dmfe_run, the main loop with its step control on the device; the
+host-driven dmfe_loop_iteration is reported beside it); all inputs resident in HBM.
 
 Prints one JSON line.  --gpus N with N > 1 is rejected: this build targets one GPU.
 """
