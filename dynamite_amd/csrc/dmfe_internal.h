@@ -32,11 +32,12 @@ constexpr int kMaxMethod = 11;  // integrators 1 RK54, 2 SSPRK104, 3.. SERK2 lev
 constexpr int kTailMaxN = 1024;
 
 // dynamic LDS of one RHS row block: [N] V[T1-1], ([N] V[T1-2], [N] D[T1-1] if N <= kTailMaxN),
-// [cw] B2 values, [cw] Hermite abscissae, [cw] time rows, [cw] stencil starts, and for K
-// tangent directions [K N] tangent last rows + [K cw] tangent B2 values
+// [cw] B2 values, [cw] Hermite abscissae, [cw] rho of the entries' next rows, [cw] time
+// rows, [cw] stencil starts, 8 bytes of alignment, and for K tangent directions [K N]
+// tangent last rows + [K cw] tangent B2 values
 inline size_t rhs_lds_bytes(int N, int cw, int K) {
     const int rows = N <= kTailMaxN ? 3 : 1;
-    return (size_t)(rows * N + cw) * 16 + (size_t)cw * (8 + 2 * 4) + (size_t)K * (N + cw) * 16;
+    return (size_t)(rows * N + cw) * 16 + (size_t)cw * (2 * 8 + 2 * 4) + 8 + (size_t)K * (N + cw) * 16;
 }
 
 // f(q) = lambda q^p + (1-lambda) q^p2 and its derivatives (math_ops.cpp:23-37)

@@ -454,7 +454,7 @@ __device__ inline double2 stencil_last(const double* __restrict__ w, long long P
 template <bool LOGR, int G>
 __device__ __forceinline__ void b2_groups_sums(const Tables& T, const History& H, const Work& W, const RhsArgs& a, int j,
                                           int c0, int C, const int* b2ix, const int* b2s, const double* b2u,
-                                          double2* b2, const double2* tail /* [3][N] or nullptr */) {
+                                          const double* b2rr, double2* b2, const double2* tail /* [3][N] or nullptr */) {
     const int N = T.N, m = T.m, T1 = a.T1;
     constexpr int GPW = kWave / G;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
@@ -524,7 +524,7 @@ __device__ __forceinline__ void b2_groups_sums(const Tables& T, const History& H
             const double u = b2u[e], u2 = u * u, u3 = u2 * u;
             double qk, qr;
             if (b2int) {
-                const double rr = H.rho[ix + 1];
+                const double rr = b2rr[e];
                 const double h0 = 1 - 3 * u2 + 2 * u3, h1 = u - 2 * u2 + u3, h2 = 3 * u2 - 2 * u3, h3 = -u2 + u3;
                 qk = h0 * y0k + h1 * e1k + h2 * y1k + h3 * e2k / rr;
                 if (!LOGR || !(y0r > 0.0 && y1r > 0.0))
@@ -579,9 +579,9 @@ __device__ __forceinline__ void b2_tap(const double (&c)[4], double wd, double2 
 template <bool LOGR, int G, int NT>
 __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, const Work& W, const RhsArgs& a, int j,
                                           int c0, int C, const int* b2ix, const int* b2s, const double* b2u,
-                                          double2* b2, const double2* tail /* [3][N] or nullptr */) {
+                                          const double* b2rr, double2* b2, const double2* tail /* [3][N] or nullptr */) {
     if constexpr (LOGR) {
-        b2_groups_sums<LOGR, G>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2, tail);
+        b2_groups_sums<LOGR, G>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tail);
         return;
     }
     const int N = T.N, m = T.m, T1 = a.T1;
@@ -598,7 +598,7 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
         const bool b2int = ix < (size_t)(T1 - 1);
         const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
         double c[4];
-        b2_hermite(b2u[ee], b2int, b2int ? H.rho[ix + 1] : 1.0, c);
+        b2_hermite(b2u[ee], b2int, b2rr[ee], c);
         double ak = 0.0, ar = 0.0;
         // the NT taps dd, dd + G, ... of a lane are loaded in one pass (one memory round
         // trip instead of one per tap); accumulation order stays dd, dd + G, ...
@@ -660,7 +660,8 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
 template <int G, int K>
 __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& H, const TanArgs& X, const RhsArgs& a,
                                                 int j, int c0, int C, const int* b2ix, const int* b2s,
-                                                const double* b2u, double2* b2, double2* tb2, int cw) {
+                                                const double* b2u, const double* b2rr, double2* b2, double2* tb2,
+                                                int cw) {
     const int N = T.N, m = T.m, T1 = a.T1;
     constexpr int GPW = kWave / G;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
@@ -685,7 +686,7 @@ __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& 
         const size_t o0 = (ix - 1) * N + sst, o1 = ix * N + sst, o2 = (b2int ? ix + 1 : ix) * N + sst;
         const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
         double c[4];
-        b2_hermite(b2u[ee], b2int, b2int ? H.rho[ix + 1] : 1.0, c);
+        b2_hermite(b2u[ee], b2int, b2rr[ee], c);
         double ak[1 + K], ar[1 + K];
 #pragma unroll
         for (int h = 0; h <= K; ++h) ak[h] = ar[h] = 0.0;
@@ -758,7 +759,8 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     double2* vlast = tail_rows ? dyn + N : dyn;
     double2* b2 = dyn + (tail_rows ? 3 * N : N);
     double* b2u = reinterpret_cast<double*>(b2 + cw);
-    int* b2ix = reinterpret_cast<int*>(b2u + cw);
+    double* b2rr = b2u + cw;  // rho[ix + 1] of an entry inside the history (1 in the last interval)
+    int* b2ix = reinterpret_cast<int*>(b2rr + cw);
     int* b2s = b2ix + cw;
     const int C = c1 - c0;
     // the B2 operands of this thread's first entry and the scalars of phase 2 are
@@ -809,7 +811,9 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
             }
         }
     }
-    double2* tvlast = reinterpret_cast<double2*>(b2s + cw);  // K > 0 only
+    // K > 0 only; b2u [cw] doubles, b2rr [cw] doubles, b2ix and b2s [cw] ints each, then
+    // 16-byte alignment for the tangent rows
+    double2* tvlast = reinterpret_cast<double2*>(b2u + 3 * cw + (cw & 1));
     double2* tb2 = tvlast + (size_t)K * N;
     if constexpr (K > 0) {
 #pragma unroll
@@ -832,12 +836,16 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
             b2ix[e] = (int)ix;
             b2s[e] = e == e1 ? s1 : T.inds[2][k];
             b2u[e] = px - ix;
+            // requested here, once per entry and overlapped with the other entries' loads,
+            // so that the B2 groups below start their gathers without a dependent load
+            b2rr[e] = ix < (size_t)(T1 - 1) ? H.rho[ix + 1] : 1.0;
         }
         __syncthreads();
         // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
         // instruction touch 64/G entries x a few contiguous cache lines
-        if constexpr (K == 0) b2_groups<LOGR, G, NT>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2, tail_rows ? rows : nullptr);
-        else b2_groups_multi<G, K>(T, H, *X, a, j, c0, C, b2ix, b2s, b2u, b2, tb2, cw);
+        if constexpr (K == 0)
+            b2_groups<LOGR, G, NT>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tail_rows ? rows : nullptr);
+        else b2_groups_multi<G, K>(T, H, *X, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tb2, cw);
     }
     __syncthreads();
 
