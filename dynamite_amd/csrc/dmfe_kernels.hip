@@ -543,16 +543,23 @@ __device__ __forceinline__ void b2_groups_sums(const Tables& T, const History& H
     }
 }
 
+// Hermite coefficient of the DY2 slice inside the history, (u^3 - u^2) / rho[ix + 1]
+// (index_mat.cpp:45-70); computed once per entry, before the gathers.
+__device__ __forceinline__ double b2_c3(double u, double rr) {
+    const double u2 = u * u, u3 = u2 * u;
+    return (-u2 + u3) / rr;
+}
+
 // Hermite coefficients of the four B2 slices (Y0, Y1, DY1, DY2) at abscissa u
-// (index_mat.cpp:45-70): cubic Hermite inside the history, the quadratic form in the last
-// interval (where DY2 does not enter).
-__device__ __forceinline__ void b2_hermite(double u, bool interior, double rr, double (&c)[4]) {
+// (index_mat.cpp:45-70): cubic Hermite inside the history (c3 from b2_c3), the quadratic
+// form in the last interval (where DY2 does not enter).
+__device__ __forceinline__ void b2_hermite(double u, bool interior, double c3, double (&c)[4]) {
     const double u2 = u * u, u3 = u2 * u;
     if (interior) {
         c[0] = 1 - 3 * u2 + 2 * u3;
         c[1] = 3 * u2 - 2 * u3;
         c[2] = u - 2 * u2 + u3;
-        c[3] = (-u2 + u3) / rr;
+        c[3] = c3;
     } else {
         c[0] = 1 - u2;
         c[1] = u2;
@@ -598,7 +605,7 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
         const bool b2int = ix < (size_t)(T1 - 1);
         const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
         double c[4];
-        b2_hermite(b2u[ee], b2int, b2rr[ee], c);
+        b2_hermite(b2u[ee], b2int, b2rr[ee], c);  // linear form: b2rr holds c3
         double ak = 0.0, ar = 0.0;
         // the NT taps dd, dd + G, ... of a lane are loaded in one pass (one memory round
         // trip instead of one per tap); accumulation order stays dd, dd + G, ...
@@ -686,7 +693,7 @@ __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& 
         const size_t o0 = (ix - 1) * N + sst, o1 = ix * N + sst, o2 = (b2int ? ix + 1 : ix) * N + sst;
         const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
         double c[4];
-        b2_hermite(b2u[ee], b2int, b2rr[ee], c);
+        b2_hermite(b2u[ee], b2int, b2rr[ee], c);  // b2rr holds c3
         double ak[1 + K], ar[1 + K];
 #pragma unroll
         for (int h = 0; h <= K; ++h) ak[h] = ar[h] = 0.0;
@@ -759,7 +766,9 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     double2* vlast = tail_rows ? dyn + N : dyn;
     double2* b2 = dyn + (tail_rows ? 3 * N : N);
     double* b2u = reinterpret_cast<double*>(b2 + cw);
-    double* b2rr = b2u + cw;  // rho[ix + 1] of an entry inside the history (1 in the last interval)
+    // per entry inside the history: the log-response form keeps rho[ix + 1], the linear form
+    // the DY2 Hermite coefficient b2_c3 (1 and 0 in the last interval)
+    double* b2rr = b2u + cw;
     int* b2ix = reinterpret_cast<int*>(b2rr + cw);
     int* b2s = b2ix + cw;
     const int C = c1 - c0;
@@ -836,9 +845,11 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
             b2ix[e] = (int)ix;
             b2s[e] = e == e1 ? s1 : T.inds[2][k];
             b2u[e] = px - ix;
-            // requested here, once per entry and overlapped with the other entries' loads,
-            // so that the B2 groups below start their gathers without a dependent load
-            b2rr[e] = ix < (size_t)(T1 - 1) ? H.rho[ix + 1] : 1.0;
+            // requested and divided here, once per entry and overlapped with the other
+            // entries' loads, so that the B2 groups below start their gathers at once
+            const bool inside = ix < (size_t)(T1 - 1);
+            const double rr = inside ? H.rho[ix + 1] : 1.0;
+            b2rr[e] = LOGR ? rr : (inside ? b2_c3(px - ix, rr) : 0.0);
         }
         __syncthreads();
         // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
