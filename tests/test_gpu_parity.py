@@ -187,3 +187,40 @@ def test_full_size_L512_steps_match_oracle():
     assert np.array_equal(b1, o1) and np.array_equal(b2, o2)
     assert abs(ctx.observables(tg)[0] - orc.energy(to)) <= 1e-11 * abs(orc.energy(to))
     ctx.close()
+
+
+def test_main_loop_to_the_stop_matches_reference_fixture():
+    """dmfe_run (device-side step control) on the L=128 quench with tmax = 1e3 against
+    the reference's own CPU main loop (tests/golden/quench128_main_loop.npz, written by
+    tests/golden/make_quench128_fixture.py from oracle/_ref, one OpenMP thread; the C
+    oracle reproduces it bit for bit, tests/test_oracle_golden_quench.py).  The loop
+    ends by delta_t < delta_t_min at loop 2061 (t = 4.037).  Bars: the same iteration
+    count and the same proposed times (the dt sequence) at every record; E(t) and
+    QK(t,0) within 1e-10 relative up to loop 1375 (t = 1.6).  After that the step is
+    near its stability limit and rounding differences grow: the reference's own run
+    with 8 OpenMP threads (another summation order) differs from its one-thread run by
+    5e-13 at t = 3 and 1e-6 at the stop, so the bar there is 1e-5."""
+    import os
+
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "quench128_main_loop.npz"))
+    N, tmax_grid, spline, order, p, p2, lam, gamma, T0, dt_min, delta_max, tmax = f["params"]
+    g = grid(int(N), tmax_grid, order=int(order), s=int(spline))
+    prm = dm.default_params(p=int(p), p2=int(p2), lambda_=lam, T0=T0, Gamma=gamma, delta_t_min=dt_min,
+                            delta_max=delta_max)
+    ctx = dm.Context(g, prm)
+    ctx.init_quench()
+    n_ref = int(f["final_control"][3])
+    print(f"L={int(N)} quench to tmax={tmax:g}: reference stops at loop {n_ref}", flush=True)
+    steps, corr = ctx.run(tmax, 10 * n_ref, corr_cap=n_ref + 16)
+    assert steps == n_ref
+    rows = corr[f["loop"] - 1]
+    assert np.array_equal(rows[:, 0], f["t"])
+    smooth = f["loop"] <= 1375
+    for col, key in ((1, "energy"), (2, "qk0")):
+        err = np.abs(rows[:, col] - f[key]) / np.abs(f[key])
+        print(f"{key}: max rel err {err[smooth].max():.2e} (loop <= 1375), {err[~smooth].max():.2e} after",
+              flush=True)
+        assert err[smooth].max() <= 1e-10, key
+        assert err[~smooth].max() <= 1e-5, key
+    assert ctx.get_control()["delta_t"] == f["final_control"][0]
+    ctx.close()
