@@ -435,8 +435,9 @@ def main():
             f"{2 * nb / ms / 1e6:.0f} GB/s")
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (got --gpus {a.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (got --gpus {a.gpus}); independent "
+              "quenches would shard across GPUs with no communication, which is not run here.",
+              file=sys.stderr)
         sys.exit(2)
     if a.sensitivity_only:
         print(json.dumps({"sensitivities": sensitivity_run()}), flush=True)

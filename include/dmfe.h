@@ -255,7 +255,7 @@ int dmfe_profile_read(dmfe_ctx* ctx, int kind, double* total_ms, int64_t* launch
  * recomputing checkpoints.  The time grid (dt sequence, positions) is frozen, i.e. the
  * derivative of the discrete map of the run that is being made.  Not available in
  * log-response mode or across a sparsification point.  The reference has no
- * counterpart (BASELINE's adjoint requirement). */
+ * counterpart: these entry points serve callers that need d(E, QK(t,0))/d(Gamma, lambda). */
 
 /* Starts K <= 2 tangent directions right after dmfe_init_quench: direction k perturbs
  * (Gamma, lambda) by (dGamma[k], dLambda[k]).  Subsequent dmfe_rk_step /
