@@ -208,9 +208,15 @@ class Context:
         return t.value
 
     def set_state(self, t1grid, QK, QR, dQK, dQR, rvec, drvec, dtratio=None):
-        arrs = [np.ascontiguousarray(a, np.float64) for a in (t1grid, QK, QR, dQK, dQR, rvec, drvec)]
-        dtr = None if dtratio is None else np.ascontiguousarray(dtratio, np.float64)
+        arrs = [np.ascontiguousarray(a, np.float64).ravel() for a in (t1grid, QK, QR, dQK, dQR, rvec, drvec)]
+        dtr = None if dtratio is None else np.ascontiguousarray(dtratio, np.float64).ravel()
         T1 = len(arrs[0])
+        # the C ABI reads T1 * N history values and T1 per-row values from each array
+        # (time_steps.cpp:55,136 reject vectors of unequal sizes the same way)
+        sizes = [a.size for a in arrs] + ([] if dtr is None else [dtr.size])
+        want = [T1] + [T1 * self.N] * 4 + [T1] * 2 + ([] if dtr is None else [T1])
+        if sizes != want:
+            raise ValueError(f"set_state: array sizes {sizes} do not match T1={T1}, N={self.N} (expected {want})")
         _check(lib().dmfe_set_state(self._h, C.c_int64(T1), *[_ptr(a) for a in arrs], _ptr(dtr)), "dmfe_set_state")
 
     def get_state(self):

@@ -772,11 +772,11 @@ extern "C" int dmfe_get_T1(const dmfe_ctx* c, int64_t* T1) {
 extern "C" int dmfe_set_state(dmfe_ctx* c, int64_t T1, const double* t1grid, const double* QK, const double* QR,
                               const double* dQK, const double* dQR, const double* rvec, const double* drvec,
                               const double* dtratio) {
-    if (c) tangent_free(c);
     if (!c || T1 < 1 || !t1grid || !QK || !QR || !dQK || !dQR || !rvec || !drvec)
         return set_error(DMFE_ERR_ARG, "dmfe_set_state: null argument or T1 < 1");
     for (int64_t i = 1; i < T1; ++i)
         if (!(t1grid[i] > t1grid[i - 1])) return set_error(DMFE_ERR_ARG, "dmfe_set_state: t1grid must be strictly increasing");
+    tangent_free(c);  // a new history ends the tangent run (a rejected call keeps it)
     int rc = ensure_rows(c, T1 + 1);
     if (rc) return rc;
     const size_t N = (size_t)c->N, n = (size_t)T1;
