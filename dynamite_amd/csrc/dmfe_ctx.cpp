@@ -828,6 +828,9 @@ extern "C" int dmfe_set_control(dmfe_ctx* c, double delta_t, double delta, doubl
     if (!c || method < DMFE_RK54 || method > kMaxMethod)
         return set_error(DMFE_ERR_ARG, "dmfe_set_control: bad argument (method 1 RK54, 2 SSPRK104, 3.." +
                                            std::to_string(kMaxMethod) + " SERK2(2 (method - 2)))");
+    if (!(delta_t > 0.0) || !std::isfinite(delta_t) || !std::isfinite(delta) || !std::isfinite(delta_old) || loop < 0)
+        return set_error(DMFE_ERR_ARG, "dmfe_set_control: delta_t must be finite and > 0, delta and delta_old finite, "
+                                       "loop >= 0");
     c->dt = delta_t; c->delta = delta; c->delta_old = delta_old; c->loop = loop; c->method = method;
     return DMFE_OK;
 }

@@ -162,7 +162,8 @@ int dmfe_save_state(dmfe_ctx* ctx, const char* path);
 int dmfe_load_state(dmfe_ctx* ctx, const char* path, double* energy_out);
 
 /* Control scalars (config.delta_t, delta, delta_old, loop, rk->init: 1 RK54, 2 SSPRK104,
- * 3..11 SERK2 with q = 2 (method - 2)). */
+ * 3..11 SERK2 with q = 2 (method - 2)); delta_t must be finite and > 0, delta and
+ * delta_old finite, loop >= 0. */
 int dmfe_set_control(dmfe_ctx* ctx, double delta_t, double delta, double delta_old, int64_t loop, int method);
 int dmfe_get_control(const dmfe_ctx* ctx, double* delta_t, double* delta, double* delta_old, int64_t* loop,
                      int* method);

@@ -63,6 +63,12 @@ def test_bad_state_and_control_leave_the_context_unchanged():
     for method in (0, 99):
         with pytest.raises(dm.DmfeError, match="method"):
             ctx.set_control(0.01, 0.0, 0.0, 0, method)
+    ctl = ctx.get_control()
+    for dt, delta, loop in ((0.0, 0.0, 0), (-0.01, 0.0, 0), (np.nan, 0.0, 0), (np.inf, 0.0, 0), (0.01, np.nan, 0),
+                            (0.01, 0.0, -1)):
+        with pytest.raises(dm.DmfeError, match="delta_t must be finite"):
+            ctx.set_control(dt, delta, 0.0, loop, dm.RK54)
+    assert ctx.get_control() == ctl
     with pytest.raises(dm.DmfeError, match="bad mode"):
         ctx.interpolate(7)
     with pytest.raises(dm.DmfeError, match="K <= 2"):
