@@ -925,6 +925,9 @@ extern "C" int dmfe_batch_set_control(dmfe_batch* b, int i, double delta_t, doub
                                       int method) {
     if (!b || i < 0 || i >= n_inst(b) || (method != DMFE_RK54 && method != DMFE_SSPRK104))
         return set_error(DMFE_ERR_ARG, "dmfe_batch_set_control: bad argument");
+    if (!(delta_t > 0.0) || !std::isfinite(delta_t) || !std::isfinite(delta) || !std::isfinite(delta_old) || loop < 0)
+        return set_error(DMFE_ERR_ARG, "dmfe_batch_set_control: delta_t must be finite and > 0, delta and delta_old "
+                                       "finite, loop >= 0");
     int rc = pull_ctl(b);
     if (rc) return rc;
     Ctl& c = b->ctl[(size_t)i];

@@ -369,6 +369,7 @@ int dmfe_batch_run(dmfe_batch* b, int64_t max_steps, int use_graph, int chunk, i
  * dmfe_get_T1 / dmfe_get_state. */
 int dmfe_batch_get_control(dmfe_batch* b, int i, double* delta_t, double* delta, double* delta_old, int64_t* loop,
                            int* method, int* status);
+/* Same checks as dmfe_set_control; methods 1 (RK54) and 2 (SSPRK104). */
 int dmfe_batch_set_control(dmfe_batch* b, int i, double delta_t, double delta, double delta_old, int64_t loop,
                            int method);
 int dmfe_batch_get_T1(dmfe_batch* b, int i, int64_t* T1);

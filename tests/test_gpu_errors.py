@@ -81,3 +81,25 @@ def test_bad_state_and_control_leave_the_context_unchanged():
     hK1, hR1 = ctx.rhs()
     assert np.array_equal(hK0, hK1) and np.array_equal(hR0, hR1)
     ctx.close()
+
+
+def test_batch_rejects_bad_control_and_keeps_the_instance():
+    g, prm = make()
+    b = dm.Batch([g, g], [prm, prm], rows_cap=64)
+    b.init_quench()
+    b.run(5, use_graph=False)
+    print("batch: bad set_control arguments", flush=True)
+    c0, s0 = b.get_control(1), b.get_state(1)
+    for dt, delta, loop, method in ((0.0, 0.0, 0, dm.RK54), (np.nan, 0.0, 0, dm.RK54), (0.01, np.inf, 0, dm.RK54),
+                                    (0.01, 0.0, -3, dm.RK54)):
+        with pytest.raises(dm.DmfeError, match="delta_t must be finite"):
+            b.set_control(1, dt, delta, 0.0, loop, method)
+    with pytest.raises(dm.DmfeError, match="bad argument"):
+        b.set_control(1, 0.01, 0.0, 0.0, 0, 7)
+    with pytest.raises(dm.DmfeError, match="bad argument"):
+        b.set_control(2, 0.01, 0.0, 0.0, 0, dm.RK54)
+    assert b.get_control(1) == c0
+    s1 = b.get_state(1)
+    for k in s0:
+        assert np.array_equal(s0[k], s1[k]), k
+    b.close()
