@@ -300,12 +300,9 @@ def batch_workload(n=64):
     """64 quenches: bath temperature Gamma ~ U[0.3, 1.0], lambda ~ U[0, 1], L drawn 50/50
     from {256, 512} (seeded), p=3, q=4, T0=inf; every instance on its own grid package
     of the bench grid family (instances of one L share the device tables)."""
-    import numpy as np
     import dynamite_amd as dm
-    rng = np.random.default_rng(BATCH_SEED)
-    Gam = rng.uniform(0.3, 1.0, n)
-    lam = rng.uniform(0.0, 1.0, n)
-    Ls = rng.permutation(np.array([256] * (n // 2) + [512] * (n - n // 2)))
+    from dynamite_amd.workloads import batch_workload_spec
+    Gam, lam, Ls = batch_workload_spec(n, BATCH_SEED)
     grids = {L: make_grid(int(L)) for L in sorted(set(Ls.tolist()))}
     gl, pl = [], []
     for i in range(n):

@@ -548,6 +548,22 @@ class Batch:
                                           _ptr(out["dtratio"])), "dmfe_batch_get_state")
         return out
 
+    def set_state(self, i, t1grid, QK, QR, dQK, dQR, rvec, drvec, dtratio=None):
+        """History of instance i (dmfe_set_state layout) + a fresh binary-search interpolation."""
+        arr = [np.ascontiguousarray(a, np.float64) for a in (t1grid, QK, QR, dQK, dQR, rvec, drvec)]
+        T1, N = len(arr[0]), self.N[i]
+        for a in arr[1:5]:
+            if a.size != T1 * N:
+                raise ValueError(f"Batch.set_state: history arrays must hold T1*N = {T1 * N} values")
+        for a in arr[5:]:
+            if a.size != T1:
+                raise ValueError("Batch.set_state: rvec and drvec must hold T1 values")
+        rho = None if dtratio is None else np.ascontiguousarray(dtratio, np.float64)
+        if rho is not None and rho.size != T1:
+            raise ValueError("Batch.set_state: dtratio must hold T1 values")
+        _check(lib().dmfe_batch_set_state(self._h, C.c_int(i), C.c_int64(T1), *[_ptr(a) for a in arr], _ptr(rho)),
+               "dmfe_batch_set_state")
+
     def record(self, cap):
         self._rec_cap = int(cap)
         _check(lib().dmfe_batch_record(self._h, C.c_int(int(cap))), "dmfe_batch_record")

@@ -456,7 +456,10 @@ int device_loop_open(const DeviceLoop& q, dmfe_batch** out) {
     b->borrowed = true;
     b->stream = q.stream;
     (void)hipGetDevice(&b->device);
-    batch_upload_tableaux(dp54(), ssprk104());
+    if (int rc = batch_upload_tableaux(dp54(), ssprk104())) {
+        delete b;
+        return rc;
+    }
     b->log_response = q.prm.log_response;
     GridDev gd;
     gd.N = N; gd.m = q.T.m; gd.P = q.T.P;
@@ -617,7 +620,7 @@ extern "C" int dmfe_batch_create(int n, const dmfe_grid* grids, const dmfe_param
     auto fail = [&](int rc) { std::string msg = dmfe_last_error(); free_batch(b); return set_error(rc, msg); };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_error(DMFE_ERR_HIP, "dmfe_batch_create: stream"));
-    batch_upload_tableaux(dp54(), ssprk104());
+    if (int rc = batch_upload_tableaux(dp54(), ssprk104())) return fail(rc);
     b->log_response = params[0].log_response;
     b->inst.resize((size_t)n);
     b->ctl.assign((size_t)n, Ctl{});
@@ -976,6 +979,63 @@ extern "C" int dmfe_batch_get_state(dmfe_batch* b, int i, double* t1grid, double
     }
     HIP_TRY(hipStreamSynchronize(b->stream));
     return DMFE_OK;
+}
+
+// History of instance i from host arrays (the dmfe_set_state layout), then a fresh
+// binary-search interpolation on it (interpolate() with no positions, as after
+// loadSimulationStateBinary, initialization.cpp:277-283).  The arena grows when T1 + 1
+// rows do not fit.  The instance runs again (status RUNNING); its control scalars stay
+// as they are (dmfe_batch_set_control sets them).
+extern "C" int dmfe_batch_set_state(dmfe_batch* b, int i, int64_t T1, const double* t1grid, const double* QK,
+                                    const double* QR, const double* dQK, const double* dQR, const double* rvec,
+                                    const double* drvec, const double* dtratio) {
+    if (!b || i < 0 || i >= n_inst(b) || T1 < 1 || !t1grid || !QK || !QR || !dQK || !dQR || !rvec || !drvec)
+        return set_error(DMFE_ERR_ARG, "dmfe_batch_set_state: bad argument");
+    if (T1 > std::numeric_limits<int>::max() / 2)
+        return set_error(DMFE_ERR_ARG, "dmfe_batch_set_state: T1 too large");
+    for (int64_t k = 1; k < T1; ++k)
+        if (!(t1grid[k] > t1grid[k - 1]))
+            return set_error(DMFE_ERR_ARG, "dmfe_batch_set_state: t1grid must be strictly increasing");
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    Inst& x = b->inst[(size_t)i];
+    Ctl& c = b->ctl[(size_t)i];
+    const size_t N = (size_t)b->grids[(size_t)x.gi].N, n = (size_t)T1;
+    if (T1 + 1 > x.cap) {
+        History h{};
+        const int64_t nc = std::max<int64_t>(2 * x.cap, T1 + 64);
+        rc = alloc_hist(h, nc, (int)N);
+        if (rc) return rc;
+        free_hist(x.H);
+        x.H = h;
+        x.cap = nc;
+        c.cap = (int)nc;
+        rc = upload_desc(b);
+        if (rc) return rc;
+    }
+    std::vector<double> rho(n, 0.0);
+    if (dtratio) {
+        rho.assign(dtratio, dtratio + n);
+    } else {  // io_input.cpp:169-172
+        for (size_t k = 2; k < n; ++k) rho[k] = (t1grid[k] - t1grid[k - 1]) / (t1grid[k - 1] - t1grid[k - 2]);
+    }
+    std::vector<double2> v(n * N), d(n * N);
+    for (size_t k = 0; k < n * N; ++k) { v[k] = make_double2(QK[k], QR[k]); d[k] = make_double2(dQK[k], dQR[k]); }
+    HIP_TRY(hipMemcpyAsync(x.H.V, v.data(), n * N * sizeof(double2), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(x.H.D, d.data(), n * N * sizeof(double2), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(x.H.t1, t1grid, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(x.H.rho, rho.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(x.H.r, rvec, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(x.H.dr, drvec, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    c.T1 = (int)T1;
+    c.t_prop = t1grid[n - 1];
+    c.status = kRunning;
+    c.active = 0;
+    rc = interpolate_inst(b, i);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return push_ctl(b);
 }
 
 // Per-step correlation records (t, E(t), QK(t,0), loop) of every instance, kept in a

@@ -79,7 +79,19 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "--fresh") o.resume = false;
         else if (a == "-S" || a == "--serk2") o.serk2 = std::string(val("-S")) != "false";
         else if (a == "--rollback") o.rollback = true;
-        else if (a == "-D" || a == "-g" || a == "-A" || a == "-w" || a == "-F" || a == "-I") (void)val(a.c_str());
+        else if (a == "-D" || a == "-g" || a == "-A" || a == "-w" || a == "-I") {
+            // accepted for command-line compatibility (config.cpp:84-162); none of them changes
+            // this path: debug output, the CPU/GPU switch, asynchronous export, the sparsify
+            // sweep count (one sweep in the CPU build, simulation_runner.cpp:187-190) and the
+            // version check of a resumed run
+            const std::string v = val(a.c_str());
+            std::fprintf(stderr, "note: option %s %s has no effect here\n", a.c_str(), v.c_str());
+        } else if (a == "-F") {
+            if (std::string(val("-F")) != "false") {
+                std::fprintf(stderr, "-F true: the tail fit (tail_fit.cpp) is not available in this build\n");
+                return false;
+            }
+        }
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return false; }
     }
     if (o.grid_dir.empty()) o.grid_dir = "Grid_data/" + std::to_string(o.len);
@@ -131,13 +143,17 @@ void write_params(const std::string& dir, const Options& o, dmfe_ctx* ctx, doubl
                  o.lambda, o.T0, o.Gamma);
     std::fprintf(f, "# Numerical Parameters\nlen = %lld\ntmax = %.16g\ndelta_t_min = %.16g\ndelta_max = %.16g\n", o.len,
                  o.tmax, o.dt_min, o.delta_max);
-    std::fprintf(f, "use_serk2 = false\nlog_response_interp = %s\ntail_fit_enabled = false\nsparsify_sweeps = 1\n",
-                 o.log_response ? "true" : "false");
+    std::fprintf(f, "use_serk2 = %s\nlog_response_interp = %s\ntail_fit_enabled = false\nsparsify_sweeps = 1\n",
+                 o.serk2 ? "true" : "false", o.log_response ? "true" : "false");
     std::fprintf(f, "maxLoop = %lld\nrmax = [3, 13, 20, 80, 180, 320, 500, 720, 980, 1280, 1620]\n\n", o.max_loop);
     std::fprintf(f, "# Current Simulation State\ncurrent_time = %.16g\ncurrent_loop = %lld\ncurrent_delta = %.16g\n",
                  t_last, (long long)loop, delta);
-    std::fprintf(f, "current_delta_t = %.16g\ncurrent_method = %s\ncurrent_t1grid_size = %lld\n", dt,
-                 method == DMFE_RK54 ? "RK54" : "SSPRK104", (long long)T1);
+    // io_save_params.cpp:234: RK54, SSPRK104, or SERK2(q) with q = 2 (method - 2)
+    const std::string mname = method == DMFE_RK54       ? std::string("RK54")
+                              : method == DMFE_SSPRK104 ? std::string("SSPRK104")
+                                                        : "SERK2(" + std::to_string(2 * (method - 2)) + ")";
+    std::fprintf(f, "current_delta_t = %.16g\ncurrent_method = %s\ncurrent_t1grid_size = %lld\n", dt, mname.c_str(),
+                 (long long)T1);
     std::fprintf(f, "current_QK0 = %.16g\ncurrent_QR0 = %.16g\ncurrent_r = %.16g\ncurrent_energy = %.16g\n",
                  qk[(T1 - 1) * o.len], qr[(T1 - 1) * o.len], r[T1 - 1], energy);
     std::fclose(f);

@@ -252,7 +252,8 @@ struct Tableau {
 const Tableau& dp54();
 const Tableau& ssprk104();
 double host_fl(const dmfe_params& p, double q, int d);
-void batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk);
+// uploads the constant-memory tableaux of the current device once (DMFE_* status)
+int batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk);
 
 // The main loop of one quench on the device-side step control (the batch machinery with
 // one instance), borrowing the caller's tables, work buffers and stream.  The history

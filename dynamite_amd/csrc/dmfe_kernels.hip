@@ -20,6 +20,8 @@
 // 64 lanes (64 consecutive columns of one row) read 512 contiguous bytes per tap.
 #include "dmfe_internal.h"
 
+#include <mutex>
+
 namespace dmfe {
 namespace {
 
@@ -2135,7 +2137,17 @@ void launch_gather_rows(const Tables& T, const History& src, const History& dst,
 }
 
 
-void batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk) {
+// The tableaux live in constant memory, once per device: uploaded by the first batch or
+// device loop created on a device (under a lock, so concurrent first calls from several
+// host threads upload once) and never rewritten while kernels read them.
+int batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk) {
+    static std::mutex mu;
+    static std::vector<char> done;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("tableaux: hipGetDevice: ") + hipGetErrorString(e));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev < done.size() && done[(size_t)dev]) return DMFE_OK;
     double A[2][kMaxStages][kMaxStages] = {}, B[2][kMaxStages] = {}, B2[2][kMaxStages] = {};
     const Tableau* t[2] = {&rk54, &ssprk};
     for (int k = 0; k < 2; ++k)
@@ -2144,9 +2156,13 @@ void batch_upload_tableaux(const Tableau& rk54, const Tableau& ssprk) {
             B[k][i] = t[k]->b[i];
             B2[k][i] = t[k]->b2[i];
         }
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(cA), A, sizeof A);
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(cB), B, sizeof B);
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(cB2), B2, sizeof B2);
+    e = hipMemcpyToSymbol(HIP_SYMBOL(cA), A, sizeof A);
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cB), B, sizeof B);
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cB2), B2, sizeof B2);
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("tableaux upload: ") + hipGetErrorString(e));
+    if (done.size() <= (size_t)dev) done.resize((size_t)dev + 1, 0);
+    done[(size_t)dev] = 1;
+    return DMFE_OK;
 }
 
 void launch_batch_begin(InstDesc* D, int n, hipStream_t s) {

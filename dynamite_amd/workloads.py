@@ -40,3 +40,17 @@ def synthetic_history(theta, T1=None, t_scale=1e3, seed=SEED, noise=1e-3):
         rho[i] = (t1[i] - t1[i - 1]) / (t1[i - 1] - t1[i - 2])
     return dict(t1grid=t1, dtratio=rho, rvec=r, drvec=dr, QK=QK.ravel(), QR=QR.ravel(), dQK=dQK.ravel(),
                 dQR=dQR.ravel())
+
+
+BATCH_SEED = 20261203
+
+
+def batch_workload_spec(n=64, seed=BATCH_SEED):
+    """Parameter points of the batched-quench workload: bath temperature Gamma ~ U[0.3, 1.0],
+    lambda ~ U[0, 1] and the grid length L drawn 50/50 from {256, 512} (a seeded
+    permutation), numpy PCG64(seed).  Returns (Gamma[n], lambda[n], L[n])."""
+    rng = np.random.default_rng(seed)
+    Gam = rng.uniform(0.3, 1.0, n)
+    lam = rng.uniform(0.0, 1.0, n)
+    Ls = rng.permutation(np.array([256] * (n // 2) + [512] * (n - n // 2)))
+    return Gam, lam, Ls

@@ -376,6 +376,16 @@ int dmfe_batch_get_T1(dmfe_batch* b, int i, int64_t* T1);
 int dmfe_batch_get_state(dmfe_batch* b, int i, double* t1grid, double* QK, double* QR, double* dQK, double* dQR,
                          double* rvec, double* drvec, double* dtratio);
 
+/* History of instance i in the dmfe_set_state layout (t1grid[T1], QK/QR/dQK/dQR
+ * [T1*N], rvec, drvec [T1]; dtratio may be NULL: rebuilt as io_input.cpp:169-172),
+ * followed by a fresh binary-search interpolation (interpolate() with no positions,
+ * initialization.cpp:277-283); the instance is set running, its control scalars are
+ * left to dmfe_batch_set_control.  Replaces, per instance, the state load of a resumed
+ * run (loadSimulationStateBinary, io_input.cpp:112-175). */
+int dmfe_batch_set_state(dmfe_batch* b, int i, int64_t T1, const double* t1grid, const double* QK, const double* QR,
+                         const double* dQK, const double* dQR, const double* rvec, const double* drvec,
+                         const double* dtratio);
+
 /* Correlation records (the correlation.txt columns t, E, QK(t,0), plus loop) of every
  * step, kept on the device in a ring of `cap` rows per instance (row (loop-1) % cap);
  * cap <= 0 switches recording off.  dmfe_batch_records copies instance i's ring

@@ -85,3 +85,54 @@ def test_dense_quench_matches_oracle():
     for key in ("QK", "QR"):
         assert np.abs(sg[key] - so[key]).max() <= 1e-11 * np.abs(so[key]).max(), key
     ctx.close()
+
+
+def test_dense_degree9_fields_and_rhs_N128_match_oracle():
+    """The configuration's dense family (B-spline degree 9, spline 5, Tmax 1e5) at N = 128."""
+    g = dm.generate_grid(128, 1e5, 5, dm.INTERP_BSPLINE, 9, 0)
+    assert g.m == 128
+    st = synthetic_history(g.theta)
+    ctx = dm.Context(g, params())
+    orc = Oracle(g, p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.8)
+    ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
+    orc.set_history(st["t1grid"], st["dtratio"], st["rvec"], st["drvec"], st["QK"], st["QR"], st["dQK"], st["dQR"])
+    ctx.interpolate(dm.POS_BSEARCH)
+    orc.interpolate(0)
+    fg, fo = ctx.fields(), orc.fields()
+    errs = {k: np.abs(fg[k] - fo[k]).max() / np.abs(fo[k]).max() for k in dm.FIELD_NAMES + ("rInt",)}
+    print("dense N=128 degree 9: fields max rel err", f"{max(errs.values()):.2e}", flush=True)
+    assert max(errs.values()) <= 1e-12, {k: f"{v:.2e}" for k, v in errs.items()}
+    hK, hR = ctx.rhs()
+    oK, oR = orc.rhs()
+    assert np.abs(hK - oK).max() <= 1e-12 * np.abs(oK).max()
+    assert np.abs(hR - oR).max() <= 1e-12 * np.abs(oR).max()
+    ctx.close()
+
+
+def test_dense_contraction_full_size_N512():
+    """The contraction at the configuration's shape: W [512^2][512] . X [512][4] (B = 4
+    right-hand sides), W seeded uniform in [-1, 1] (the kernel does not depend on the
+    values; the three stencil maps share the one table to keep host memory at 1 GB)."""
+    from numpy.lib.stride_tricks import as_strided
+
+    N, B = 512, 4
+    g0 = dm.generate_grid(N, 1e5, 5, dm.INTERP_POLY, 9, 0)
+    rng = np.random.default_rng(20261203)
+    W = rng.uniform(-1.0, 1.0, size=N * N * N)
+    g = dm.GridPackage.__new__(dm.GridPackage)
+    g.N, g.m = N, N
+    g.theta, g.integ, g.phi2 = g0.theta, g0.integ, g0.phi2
+    g.inds = np.full((3, N * N), -1, np.int64)
+    g.weights = as_strided(W, shape=(3, N * N * N), strides=(0, 8), writeable=False)
+    g.phi1 = g.pos = g.spec = None
+    ctx = dm.Context(g, params(), rows_hint=8)
+    X = rng.uniform(-1.0, 1.0, size=(N, B))
+    Wm = W.reshape(N * N, N)
+    ref = Wm @ X
+    bound = 4 * N * 2.2e-16 * (np.abs(Wm) @ np.abs(X))
+    for map_ in (0, 2):
+        Y, ms = dm.dense_apply(ctx, X, map_)
+        err = np.max(np.abs(Y - ref) / (bound + 1e-300))
+        print(f"dense N=512 B=4 map {map_}: {1e3 * ms:.1f} us, max |Y-ref| / bound = {err:.3f}", flush=True)
+        assert np.all(np.abs(Y - ref) <= bound + 1e-300), err
+    ctx.close()
