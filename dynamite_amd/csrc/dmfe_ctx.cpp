@@ -155,6 +155,8 @@ struct dmfe_ctx {
     int64_t loop = 0;
     int method = DMFE_RK54;
     bool interp_valid = false;
+    double* scratch = nullptr;  // this device's scratch array (dmfe_rhs_noalloc), resolved at creation
+    std::string scratch_msg;    // why it is unavailable
     int fcur = 0;  // Floor/diff buffer of the latest interpolation
     // optional per-kernel timing with HIP events on the context stream
     bool profiling = false;
@@ -671,6 +673,9 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
     c->specRad = 4 * std::sqrt(host_fl(*prm, 1, 2));
     auto fail = [&](int code, const std::string& msg) { free_ctx(c); return set_error(code, msg); };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(DMFE_ERR_HIP, "stream create failed");
+    // the no-allocation entry point's scratch on this device (looked up once per device);
+    // a failure only disables dmfe_rhs_noalloc, which then reports it
+    if (scratch_address(&c->scratch) != DMFE_OK) c->scratch_msg = dmfe_last_error();
     const size_t bN = (size_t)N * sizeof(double), bP = (size_t)P * sizeof(double);
     bool ok = hipMalloc(&c->theta, bN) == hipSuccess && hipMalloc(&c->integ, bN) == hipSuccess &&
               hipMalloc(&c->phi2, bP) == hipSuccess;
@@ -904,6 +909,59 @@ extern "C" int dmfe_rhs(dmfe_ctx* c, double* hK, double* hR) {
     rc = copy_out(c, hK, c->W.hK, (size_t)c->N * sizeof(double));
     if (rc) return rc;
     return copy_out(c, hR, c->W.hR, (size_t)c->N * sizeof(double));
+}
+
+// dmfe_rhs with its scratch (quadrature partials, QK_B1[j,0], rInt and the hK / hR
+// staging) in the library's file-scope device array instead of the context's work
+// buffers: no device memory is allocated or freed.  Same kernels and launch shape as
+// dmfe_rhs, hence the same results.  Every call on a device shares that device's array,
+// so calls that overlap on one device (contexts driven from several host threads) race.
+// Nothing in the array has to start at zero: the RHS kernel writes every partial it sums.
+extern "C" int dmfe_rhs_noalloc(dmfe_ctx* c, double* hK, double* hR) {
+    if (!c || !hK || !hR || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_rhs_noalloc: bad argument");
+    const size_t need = rhs_scratch_doubles(c->N, c->split);
+    if (need > kScratchDoubles)
+        return set_error(DMFE_ERR_ARG, "dmfe_rhs_noalloc: N = " + std::to_string(c->N) + " with split " +
+                                           std::to_string(c->split) + " needs " + std::to_string(need) +
+                                           " doubles of scratch, more than the " + std::to_string(kScratchDoubles) +
+                                           " available (8 N split + 4 N must fit)");
+    if (!c->scratch) return set_error(DMFE_ERR_HIP, "dmfe_rhs_noalloc: device scratch unavailable: " + c->scratch_msg);
+    Work W = c->W;
+    double* p = c->scratch;
+    W.partial = p; p += (size_t)c->N * c->split * kSums;
+    W.qkb1c0 = p; p += c->N;
+    W.rInt = p; p += c->N;
+    W.hK = p; p += c->N;
+    W.hR = p;
+    RhsArgs a{};
+    a.T1 = (int)c->T1;
+    a.tl = c->t1[c->T1 - 1];
+    a.split = c->split;
+    a.log_response = c->params.log_response;
+    a.fields = nullptr;
+    a.fbuf = c->fcur;
+    a.G = c->G;
+    if (c->dense_mfma) {  // A1/A2 of a dense grid into the context's field buffers, as dmfe_rhs
+        const double* last = reinterpret_cast<const double*>(c->H.V + (size_t)(c->T1 - 1) * c->N);
+        for (int f = 0; f < 2; ++f)
+            launch_dense_apply(c->w[f], last, reinterpret_cast<double*>(W.fA[f]), c->P, c->N, 2, 2, c->stream);
+    }
+    launch_rhs_rows(c->tables(), c->H, W, c->model, a, c->stream);
+    int rc = launch_check("rhs_rows_kernel");
+    if (rc) return rc;
+    FinishArgs f{};
+    f.mode = 0;
+    f.stage = 0;
+    f.split = c->split;
+    f.lr = (int)(c->T1 - 1);
+    f.t_rhs = c->t1[c->T1 - 1];
+    f.fbuf = c->fcur;
+    launch_stage(c->tables(), c->H, W, c->model, nullptr, &f, c->stream);
+    rc = launch_check("stage_kernel");
+    if (rc) return rc;
+    rc = copy_out(c, hK, W.hK, (size_t)c->N * sizeof(double));
+    if (rc) return rc;
+    return copy_out(c, hR, W.hR, (size_t)c->N * sizeof(double));
 }
 
 extern "C" int dmfe_rstep(dmfe_ctx* c, double* r) {

@@ -198,6 +198,14 @@ void launch_gather_rows(const Tables& T, const History& src, const History& dst,
 
 void launch_hbm_copy(const double2* src, double2* dst, long long n, hipStream_t s);
 
+// Device scratch of the no-allocation RHS entry point (dmfe_rhs_noalloc): kScratchDoubles
+// doubles (1 MiB) in a file-scope device array, one per device.  scratch_address() gives
+// its address on the current device (resolved once per device, then cached).
+constexpr size_t kScratchDoubles = 131072;
+int scratch_address(double** out);
+// doubles of scratch dmfe_rhs_noalloc needs: partials [N][split][8], QK_B1[j,0], rInt, hK, hR [N]
+inline size_t rhs_scratch_doubles(int N, int split) { return (size_t)N * split * kSums + 4 * (size_t)N; }
+
 // ---- batched quenches with device-side step control ----------------------------------
 enum CtlStatus { kRunning = 0, kDone = 1, kNeedMaint = 2, kNeedRows = 3 };
 

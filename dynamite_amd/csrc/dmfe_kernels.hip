@@ -2007,6 +2007,35 @@ __global__ __launch_bounds__(256) void hbm_copy_kernel(const double2* __restrict
 
 }  // namespace
 
+// Device scratch of dmfe_rhs_noalloc (quadrature partials, QK_B1[j,0], rInt, hK, hR):
+// a file-scope array that every device holds until the process ends.  In host code
+// `scratch` only stands in for the symbol; its device address is resolved once per
+// device with hipGetSymbolAddress (scratch_address) and only that address is handed to
+// kernels and copies.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+int scratch_address(double** out) {
+    static std::mutex mu;
+    static std::vector<double*> addr;
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("scratch: hipGetDevice: ") + hipGetErrorString(e));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev < addr.size() && addr[(size_t)dev]) {
+        *out = addr[(size_t)dev];
+        return DMFE_OK;
+    }
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_error(DMFE_ERR_HIP, std::string("scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    if (addr.size() <= (size_t)dev) addr.resize((size_t)dev + 1, nullptr);
+    addr[(size_t)dev] = static_cast<double*>(ptr);
+    *out = addr[(size_t)dev];
+    return DMFE_OK;
+}
+
 void launch_dense_apply(const double* W, const double* X, double* Y, long long P, int K, int B, int ldy,
                         hipStream_t s) {
     const long long nb = (P + kDenseRows - 1) / kDenseRows;

@@ -188,6 +188,15 @@ int dmfe_get_fields(dmfe_ctx* ctx, double* out);
  * interpolation state: hK, hR [N]. */
 int dmfe_rhs(dmfe_ctx* ctx, double* hK, double* hR);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops): dmfe_rhs with its scratch (the
+ * quadrature partials, QK_B1[j,0], rInt and the hK/hR staging) in a file-scope device
+ * array of the library (1 MiB per device, held until the process ends) instead of the
+ * context's buffers; it calls no hipMalloc/hipFree.  Limit: 8*N*split + 4*N <= 131072
+ * doubles, else DMFE_ERR_ARG.  All calls on one device share that device's array:
+ * calls that overlap on one device race.  Results equal dmfe_rhs's. */
+int dmfe_rhs_noalloc(dmfe_ctx* ctx, double* hK, double* hR);
+
 /* rstep() (time_steps.cpp:83-92) on the last history row. */
 int dmfe_rstep(dmfe_ctx* ctx, double* r);
 
