@@ -919,11 +919,11 @@ extern "C" int dmfe_rhs(dmfe_ctx* c, double* hK, double* hR) {
 // Nothing in the array has to start at zero: the RHS kernel writes every partial it sums.
 extern "C" int dmfe_rhs_noalloc(dmfe_ctx* c, double* hK, double* hR) {
     if (!c || !hK || !hR || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_rhs_noalloc: bad argument");
-    const size_t need = rhs_scratch_doubles(c->N, c->split);
-    if (need > kScratchDoubles)
+    const size_t need = rhs_scratch_doubles(c->N, c->split), avail = kScratchDoubles;
+    if (need > avail)
         return set_error(DMFE_ERR_ARG, "dmfe_rhs_noalloc: N = " + std::to_string(c->N) + " with split " +
                                            std::to_string(c->split) + " needs " + std::to_string(need) +
-                                           " doubles of scratch, more than the " + std::to_string(kScratchDoubles) +
+                                           " doubles of scratch, more than the " + std::to_string(avail) +
                                            " available (8 N split + 4 N must fit)");
     if (!c->scratch) return set_error(DMFE_ERR_HIP, "dmfe_rhs_noalloc: device scratch unavailable: " + c->scratch_msg);
     Work W = c->W;

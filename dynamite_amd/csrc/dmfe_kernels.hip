@@ -357,13 +357,22 @@ __device__ __forceinline__ void search_theta_row(const Tables& T, const Work& W,
 // ---- fused RHS rows ----------------------------------------------------------------
 constexpr int kRowThreads = 256;
 
-// Consecutive workgroups are dispatched round-robin over the 8 XCDs; give every
-// XCD a contiguous range of logical blocks (adjacent rows share history rows and
-// weight cache lines in that XCD's L2).
-__device__ inline int xcd_remap(int b, int nb) {
+// Consecutive workgroups are dispatched round-robin over the 8 XCDs.  Logical blocks
+// (row-major (row j, chunk), or the batch's launch map) are dealt to the XCDs in stripes
+// of S consecutive blocks, stripe k to XCD k mod 8: a stripe's rows, which share history
+// rows and cache lines, stay in one XCD's L2, and every XCD gets stripes from the whole
+// row range.  The B2 work of a row falls steeply with theta_j (small theta_j: targets
+// spread over the whole history; theta_j near 1: mostly the last time interval), so a
+// contiguous row range per XCD (S = nb / 8) leaves some XCDs with most of the gathers.
+// S is the largest of 16, 8, 4, 2, 1 that divides nb / 8 (the identity if 8 does not
+// divide nb).
+__device__ inline int xcd_stripe(int b, int nb) {
     constexpr int X = 8;
     if (nb % X) return b;
-    return (b % X) * (nb / X) + b / X;
+    int S = 16;
+    while (S > 1 && nb % (X * S)) S >>= 1;
+    const int x = b % X, q = b / X;
+    return ((q / S) * X + x) * S + q % S;
 }
 
 // ---- forward-mode sensitivities with respect to Gamma and lambda -----------------------
@@ -409,17 +418,30 @@ __device__ inline double rint_at(const History& H, const int* Floor, const doubl
     return (1 - d2) * H.r[f - 1] + d2 * H.r[f] - (d + d2) * H.dr[f];
 }
 
+// taps whose weights one lane requests in one memory round trip (the stencil width of the
+// default poly order 9)
+constexpr int kTapBatch = 10;
+
 #pragma clang fp contract(off)  // the compensated sum below needs unfused products
 template <bool LOGR>
 __device__ inline double2 stencil_last(const double* __restrict__ w, long long P, long long k, int m, int s,
                                        const double2* __restrict__ vl) {
     double a = 0.0, b = 0.0;
     if (!LOGR) {
-        for (int d = 0; d < m; ++d) {
-            const double wd = w[(long long)d * P + k];
-            const double2 v = vl[s + d];
-            a = a + wd * v.x;
-            b = b + wd * v.y;
+        // the weights of up to kTapBatch taps are requested together (one memory round
+        // trip per batch instead of one per few taps); accumulation order d = 0, 1, ...
+        for (int d0 = 0; d0 < m; d0 += kTapBatch) {
+            double wd[kTapBatch];
+#pragma unroll
+            for (int t = 0; t < kTapBatch; ++t) wd[t] = d0 + t < m ? w[(long long)(d0 + t) * P + k] : 0.0;
+#pragma unroll
+            for (int t = 0; t < kTapBatch; ++t) {
+                if (d0 + t < m) {
+                    const double2 v = vl[s + d0 + t];
+                    a = a + wd[t] * v.x;
+                    b = b + wd[t] * v.y;
+                }
+            }
         }
         return make_double2(a, b);
     }
@@ -654,6 +676,90 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
     }
 }
 
+// B2 (index_mat.cpp:18-91), linear form, on the block's entry lists: slots [0, nh) hold
+// the entries whose four slices are gathered from the history, slots [nh, C) those of
+// the last time interval (their rows are the block's LDS copies `tail`), so every
+// iteration of a wave gathers one kind only.  G lanes per entry, lane d holding taps d,
+// d+G, ... (NT per pass, loaded in one memory round trip together with rho[ix+1]); the
+// entry's time Hermite is folded into its taps and a group closes two sums (QK, QR).
+// Per entry the arithmetic and its order are those of b2_groups.
+template <int G, int NT>
+__device__ __forceinline__ void b2_lists(const Tables& T, const History& H, const RhsArgs& a, int j, int c0, int C,
+                                         int nh, const int* lix, const int* ls, const double* lu, const int* le,
+                                         double2* b2, const double2* tail /* [3][N] or nullptr */) {
+    const int N = T.N, m = T.m, T1 = a.T1;
+    constexpr int GPW = kWave / G;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int g = lane / G, d0 = lane % G;
+    constexpr int kWaves = kRowThreads / kWave;
+    for (int k0 = wid * GPW; k0 < nh; k0 += GPW * kWaves) {
+        const int k = k0 + g;
+        const bool ev = k < nh;
+        const int kk = ev ? k : k0;  // a valid slot: k0 < nh
+        const int e = le[kk];
+        const size_t ix = (size_t)lix[kk];
+        const int sst = ls[kk];
+        const double u = lu[kk];
+        const bool b2int = ix < (size_t)(T1 - 1);
+        const double* wk = T.w[2] + ((long long)j * N + c0 + e) * m;
+        const double2* R0 = H.V + (ix - 1) * N + sst;
+        const double2* R1 = H.V + ix * N + sst;
+        const double2* S1 = H.D + ix * N + sst;
+        const double2* S2 = H.D + (b2int ? ix + 1 : ix) * N + sst;
+        const double rr = b2int ? H.rho[ix + 1] : 1.0;
+        double ak = 0.0, ar = 0.0;
+        if (ev) {
+            for (int dd = d0; dd < m; dd += NT * G) {
+                double w[NT];
+                double2 v0[NT], v1[NT], g1[NT], g2[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int d = dd + t * G;
+                    w[t] = 0.0;
+                    v0[t] = v1[t] = g1[t] = g2[t] = make_double2(0.0, 0.0);
+                    if (d < m) {
+                        w[t] = wk[d];
+                        v0[t] = R0[d]; v1[t] = R1[d]; g1[t] = S1[d]; g2[t] = S2[d];
+                    }
+                }
+                double c[4];
+                b2_hermite(u, b2int, b2int ? b2_c3(u, rr) : 0.0, c);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    if (dd + t * G < m) b2_tap(c, w[t], v0[t], v1[t], g1[t], g2[t], ak, ar);
+            }
+        }
+        const double qk = group_sum<G>(ak), qr = group_sum<G>(ar);
+        if (d0 == 0 && ev) b2[e] = make_double2(qk, qr);
+    }
+    if (!tail) return;
+    // last interval: one lane per entry (its weights are contiguous, its taps in LDS), so
+    // all of a block's last-interval entries need one memory round trip (the weights)
+    for (int k = nh + (int)threadIdx.x; k < C; k += kRowThreads) {
+        const int e = le[k];
+        const int sst = ls[k];
+        const double* wk = T.w[2] + ((long long)j * N + c0 + e) * m;
+        double c[4];
+        b2_hermite(lu[k], false, 0.0, c);
+        const double2 *L0 = tail + sst, *L1 = tail + N + sst, *L2 = tail + 2 * N + sst;
+        double ak = 0.0, ar = 0.0;
+        for (int d0 = 0; d0 < m; d0 += kTapBatch) {
+            double w[kTapBatch];
+#pragma unroll
+            for (int t = 0; t < kTapBatch; ++t) w[t] = d0 + t < m ? wk[d0 + t] : 0.0;
+#pragma unroll
+            for (int t = 0; t < kTapBatch; ++t) {
+                const int d = d0 + t;
+                if (d < m) {
+                    const double2 h1 = L2[d];
+                    b2_tap(c, w[t], L0[d], L1[d], h1, h1, ak, ar);
+                }
+            }
+        }
+        b2[e] = make_double2(ak, ar);
+    }
+}
+
 // Row j, columns [c0, c1): the eight quadratures of QKstep/QRstep.
 //  phase 1 (B2, index_mat.cpp:18-91): one group of G lanes per target entry, lane d
 //          holding stencil tap d, so the four history-row gathers of a wave touch a
@@ -773,6 +879,8 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     double* b2rr = b2u + cw;
     int* b2ix = reinterpret_cast<int*>(b2rr + cw);
     int* b2s = b2ix + cw;
+    // linear form (K = 0): the entry of each list slot, in the (unused) Hermite-rho array
+    int* b2e = reinterpret_cast<int*>(b2rr);
     const int C = c1 - c0;
     // the B2 operands of this thread's first entry and the scalars of phase 2 are
     // requested first, so their latency overlaps the row copies below
@@ -834,7 +942,61 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
 
     // ---- phase 1: B2 ----------------------------------------------------------------
     const double c00 = H.V[0].x, c01 = H.V[0].y;
-    if (live) {
+    if constexpr (K == 0 && !LOGR) {
+        if (live) {
+            // 1a: per entry (lane per entry, coalesced): time row, stencil start, Hermite
+            // abscissa; the entries are sorted into two lists, those whose slices are
+            // gathered from the history (front of the list arrays) and those of the last
+            // time interval, read from the block's LDS rows (back, in reverse order)
+            size_t cap_hi = (size_t)(pmax - 0.5);
+            if (cap_hi > (size_t)(T1 - 1)) cap_hi = (size_t)(T1 - 1);  // memory guard
+            int* cnt = reinterpret_cast<int*>(red);                    // per-wave list counts
+            const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+            const unsigned long long below = (1ull << lane) - 1ull;
+            int nh = 0, nl = 0;
+            for (int r0 = 0; r0 < C; r0 += kRowThreads) {
+                const int e = r0 + threadIdx.x;
+                int ix = 1, sst = 0;
+                double u = 0.0;
+                if (e < C) {
+                    const long long k = (long long)j * N + c0 + e;
+                    const double px = e == e1 ? px1 : W.posB2[k];
+                    size_t x = (size_t)px;
+                    if (cap_hi < x) x = cap_hi;
+                    if (x < 1) x = 1;
+                    ix = (int)x;
+                    sst = e == e1 ? s1 : T.inds[2][k];
+                    u = px - x;
+                }
+                const bool lastrow = tail_rows && ix >= T1 - 1;
+                const bool toH = e < C && !lastrow, toL = e < C && lastrow;
+                const unsigned long long mh = __ballot(toH), ml = __ballot(toL);
+                if (lane == 0) {
+                    cnt[wid] = __popcll(mh);
+                    cnt[kRowThreads / kWave + wid] = __popcll(ml);
+                }
+                __syncthreads();
+                int oh = nh, ol = nl;
+                for (int w = 0; w < kRowThreads / kWave; ++w) {
+                    const int ch = cnt[w], cl = cnt[kRowThreads / kWave + w];
+                    if (w < wid) { oh += ch; ol += cl; }
+                    nh += ch;
+                    nl += cl;
+                }
+                if (toH || toL) {
+                    const int slot = toH ? oh + __popcll(mh & below) : C - 1 - (ol + __popcll(ml & below));
+                    b2ix[slot] = ix;
+                    b2s[slot] = sst;
+                    b2u[slot] = u;
+                    b2e[slot] = e;
+                }
+                __syncthreads();  // cnt is rewritten by the next round
+            }
+            // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
+            // instruction touch 64/G entries x a few contiguous cache lines
+            b2_lists<G, NT>(T, H, a, j, c0, C, nh, b2ix, b2s, b2u, b2e, b2, tail_rows ? rows : nullptr);
+        }
+    } else if (live) {
         // 1a: per entry (lane per entry, coalesced): time row, stencil start, Hermite abscissa
         size_t cap_hi = (size_t)(pmax - 0.5);
         if (cap_hi > (size_t)(T1 - 1)) cap_hi = (size_t)(T1 - 1);  // memory guard
@@ -1014,7 +1176,7 @@ template <bool LOGR, int G, int NT>
 __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History H, Work W, Model M, RhsArgs a) {
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int lb = xcd_stripe(blockIdx.x, gridDim.x);
     rhs_rows_body<LOGR, G, 0, NT>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red);
 }
 
@@ -1024,7 +1186,7 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_tan_kernel(Tables T, His
                                                                    TanArgs X) {
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int lb = xcd_stripe(blockIdx.x, gridDim.x);
     rhs_rows_body<false, G, K>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red, &X);
 }
 
@@ -1619,7 +1781,7 @@ __global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* 
                                                                 const int4* __restrict__ map, int n) {
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
-    const int4 mp = map[xcd_remap(blockIdx.x, gridDim.x)];
+    const int4 mp = map[xcd_stripe(blockIdx.x, gridDim.x)];
     if (mp.x < 0) return;
     const InstDesc& d = D[mp.x];
     const Ctl& c = *d.ctl;
@@ -1638,7 +1800,7 @@ __global__ __launch_bounds__(kRowThreads) void loop_rhs_kernel(InstDesc d, int n
     const int active = c.active, S = c.S;
     const RhsArgs a = step_rhs_args(c, d, n, LOGR);  // loaded with the control word
     if (!active || n >= S) return;
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int lb = xcd_stripe(blockIdx.x, gridDim.x);
     rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red);
 }
 
@@ -1651,7 +1813,7 @@ __global__ __launch_bounds__(kRowThreads) void loop_rhs_tan_kernel(InstDesc d, i
     const int active = c.active, S = c.S;
     const RhsArgs a = step_rhs_args(c, d, n, 0);  // loaded with the control word
     if (!active || n >= S) return;
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int lb = xcd_stripe(blockIdx.x, gridDim.x);
     rhs_rows_body<false, G, K>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red, &d.X);
 }
 
