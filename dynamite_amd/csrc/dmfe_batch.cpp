@@ -423,8 +423,8 @@ int upload_grid(GridDev& gd, const dmfe_grid* g) {
 
 int alloc_work(Work& W, int N, int split) {
     const size_t bN = (size_t)N * sizeof(double), bP = (size_t)N * N * sizeof(double);
-    HIP_TRY(hipMalloc(&W.posB1, bN));
-    HIP_TRY(hipMalloc(&W.posB2, bP));
+    HIP_TRY(hipMalloc(&W.posB1, 2 * bN));  // double-buffered (Work::posB1)
+    HIP_TRY(hipMalloc(&W.posB2, 2 * bP));
     HIP_TRY(hipMalloc(&W.saveB1, 3 * bN));
     HIP_TRY(hipMalloc(&W.saveB2, 3 * bP));
     HIP_TRY(hipMalloc(&W.Floor, 2 * (size_t)N * sizeof(int)));

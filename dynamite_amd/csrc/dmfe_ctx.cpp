@@ -682,7 +682,7 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
     for (int f = 0; ok && f < 3; ++f)
         ok = hipMalloc(&c->inds[f], (size_t)P * sizeof(int)) == hipSuccess &&
              hipMalloc(&c->w[f], bP * (size_t)m) == hipSuccess;
-    ok = ok && hipMalloc(&c->W.posB1, bN) == hipSuccess && hipMalloc(&c->W.posB2, bP) == hipSuccess &&
+    ok = ok && hipMalloc(&c->W.posB1, 2 * bN) == hipSuccess && hipMalloc(&c->W.posB2, 2 * bP) == hipSuccess &&
          hipMalloc(&c->W.saveB1, 3 * bN) == hipSuccess && hipMalloc(&c->W.saveB2, 3 * bP) == hipSuccess &&
          hipMalloc(&c->W.Floor, 2 * (size_t)N * sizeof(int)) == hipSuccess && hipMalloc(&c->W.diff, 2 * bN) == hipSuccess &&
          hipMalloc(&c->W.rInt, bN) == hipSuccess &&
@@ -719,7 +719,9 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
     // posB1xOld = 1, posB2xOld = 0 (initialization.cpp:234-235)
     std::vector<double> ones((size_t)N, 1.0), zeros((size_t)P, 0.0);
     up = up && hipMemcpy(c->W.posB1, ones.data(), bN, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->W.posB1 + N, ones.data(), bN, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(c->W.posB2, zeros.data(), bP, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->W.posB2 + P, zeros.data(), bP, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemset(c->W.scal, 0, 8 * sizeof(double)) == hipSuccess;
     if (!up) return fail(DMFE_ERR_HIP, "dmfe_ctx_create: upload failed");
     int rc = ensure_rows(c, std::max<int64_t>(rows_hint, 64));
@@ -864,15 +866,19 @@ extern "C" int dmfe_interpolate(dmfe_ctx* c, int mode) {
 
 extern "C" int dmfe_get_positions(dmfe_ctx* c, double* posB1, double* posB2) {
     if (!c) return set_error(DMFE_ERR_ARG, "null context");
-    if (posB1) { int rc = copy_out(c, posB1, c->W.posB1, (size_t)c->N * sizeof(double)); if (rc) return rc; }
-    if (posB2) { int rc = copy_out(c, posB2, c->W.posB2, (size_t)c->P * sizeof(double)); if (rc) return rc; }
+    const double* b1 = c->W.posB1 + (size_t)c->fcur * c->N;  // the latest interpolation's buffer
+    const double* b2 = c->W.posB2 + (size_t)c->fcur * c->P;
+    if (posB1) { int rc = copy_out(c, posB1, b1, (size_t)c->N * sizeof(double)); if (rc) return rc; }
+    if (posB2) { int rc = copy_out(c, posB2, b2, (size_t)c->P * sizeof(double)); if (rc) return rc; }
     return DMFE_OK;
 }
 
 extern "C" int dmfe_set_positions(dmfe_ctx* c, const double* posB1, const double* posB2) {
     if (!c || !posB1 || !posB2) return set_error(DMFE_ERR_ARG, "null argument");
-    HIP_TRY(hipMemcpyAsync(c->W.posB1, posB1, (size_t)c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->W.posB2, posB2, (size_t)c->P * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->W.posB1 + (size_t)c->fcur * c->N, posB1, (size_t)c->N * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->W.posB2 + (size_t)c->fcur * c->P, posB2, (size_t)c->P * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return DMFE_OK;
 }

@@ -66,8 +66,8 @@ struct History {
 };
 
 struct Work {
-    double* posB1;    // [N]  posB1xOld
-    double* posB2;    // [P]  posB2xOld
+    double* posB1;    // [2][N]  posB1xOld, buffer fbuf of the interpolation that wrote it
+    double* posB2;    // [2][P]  posB2xOld (double-buffered as Floor / diff)
     double* saveB1;   // [3][N] SSPRK stage positions
     double* saveB2;   // [3][P]
     int* Floor;       // [2][N]  B1 time rows (double-buffered: a launch's search writes

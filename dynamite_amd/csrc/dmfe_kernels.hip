@@ -295,15 +295,22 @@ __device__ __forceinline__ void search_row(const ListView& list, double tl, cons
     }
 }
 
-// B2 positions of phi2 row k (interpolation_core.cpp:27-29).
+// B2 positions of phi2 row k (interpolation_core.cpp:27-29).  Positions are double
+// buffered like Floor / diff: an interpolation writes buffer a.fbuf and seeds the
+// interpolation search from the other one, which holds the previous interpolation's
+// positions (posB2xOld) and may still be read by an RHS of the previous stage.
 template <int EPT>
 __device__ __forceinline__ void search_phi2_row(const Tables& T, const Work& W, const SearchArgs& a, int k, double* lds) {
     const int N = T.N;
-    double* out = W.posB2 + (size_t)k * N;
+    double* out = W.posB2 + (size_t)a.fbuf * T.P + (size_t)k * N;
+    const double* old = W.posB2 + (size_t)(1 - a.fbuf) * T.P + (size_t)k * N;
     double* out2 = a.saveB2 ? a.saveB2 + (size_t)k * N : nullptr;
-    if (a.mode == kSearchSame) {
-        if (out2)
-            for (int l = threadIdx.x; l < N; l += kStageThreads) out2[l] = out[l];
+    if (a.mode == kSearchSame) {  // the previous positions again (interpolation_core.cpp:25-29, same = true)
+        for (int l = threadIdx.x; l < N; l += kStageThreads) {
+            const double v = old[l];
+            out[l] = v;
+            if (out2) out2[l] = v;
+        }
         return;
     }
     if (a.mode == kSearchCopy) {
@@ -315,7 +322,7 @@ __device__ __forceinline__ void search_phi2_row(const Tables& T, const Work& W, 
         return;
     }
     const ListView list{a.t1, (size_t)a.T1, a.tl};
-    search_row<EPT>(list, a.tl, T.phi2 + (size_t)k * N, out, out, out2, N, a.mode, lds);
+    search_row<EPT>(list, a.tl, T.phi2 + (size_t)k * N, old, out, out2, N, a.mode, lds);
 }
 
 // B1 positions of the theta row and the B1 prologue Floor / diff
@@ -323,12 +330,15 @@ __device__ __forceinline__ void search_phi2_row(const Tables& T, const Work& W, 
 template <int EPT>
 __device__ __forceinline__ void search_theta_row(const Tables& T, const Work& W, const SearchArgs& a, double* lds, double* red) {
     const int N = T.N;
-    double* pos = W.posB1;
+    double* pos = W.posB1 + (size_t)a.fbuf * N;  // double buffered as the B2 positions
+    const double* old = W.posB1 + (size_t)(1 - a.fbuf) * N;
     if (a.mode == kSearchCopy) {
         for (int l = threadIdx.x; l < N; l += kStageThreads) pos[l] = a.srcB1[l];
-    } else if (a.mode != kSearchSame) {
+    } else if (a.mode == kSearchSame) {
+        for (int l = threadIdx.x; l < N; l += kStageThreads) pos[l] = old[l];
+    } else {
         const ListView list{a.t1, (size_t)a.T1, a.tl};
-        search_row<EPT>(list, a.tl, T.theta, pos, pos, nullptr, N, a.mode, lds);
+        search_row<EPT>(list, a.tl, T.theta, old, pos, nullptr, N, a.mode, lds);
     }
     __syncthreads();
     if (a.saveB1)
@@ -888,10 +898,11 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     const long long k1 = (long long)j * N + c0 + e1;
     double px1 = 0.0, pmax = 0.0;
     int s1 = 0;
+    const double* posB2 = W.posB2 + (size_t)a.fbuf * P;  // positions of the current interpolation
     if (live) {
-        pmax = W.posB2[P - 1];
+        pmax = posB2[P - 1];
         if (e1 < C) {
-            px1 = W.posB2[k1];
+            px1 = posB2[k1];
             s1 = T.inds[2][k1];
         }
     }
@@ -960,7 +971,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
                 double u = 0.0;
                 if (e < C) {
                     const long long k = (long long)j * N + c0 + e;
-                    const double px = e == e1 ? px1 : W.posB2[k];
+                    const double px = e == e1 ? px1 : posB2[k];
                     size_t x = (size_t)px;
                     if (cap_hi < x) x = cap_hi;
                     if (x < 1) x = 1;
@@ -1002,7 +1013,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
         if (cap_hi > (size_t)(T1 - 1)) cap_hi = (size_t)(T1 - 1);  // memory guard
         for (int e = e1; e < C; e += kRowThreads) {
             const long long k = (long long)j * N + c0 + e;
-            const double px = e == e1 ? px1 : W.posB2[k];
+            const double px = e == e1 ? px1 : posB2[k];
             size_t ix = (size_t)px;
             if (cap_hi < ix) ix = cap_hi;
             if (ix < 1) ix = 1;
@@ -1742,7 +1753,7 @@ __device__ __forceinline__ void stage_role_body(const InstDesc& d, int role, int
     const SearchArgs a = c.srch[n];
     if (!active || n > S || a.mode < 0) return;
     if (role == -1) search_theta_row<EPT>(d.T, d.W, a, lds, red);
-    else if (!(a.mode == kSearchSame && !a.saveB2)) search_phi2_row<EPT>(d.T, d.W, a, role, lds);
+    else search_phi2_row<EPT>(d.T, d.W, a, role, lds);
 }
 
 template <int EPT>
@@ -2267,8 +2278,8 @@ void launch_stage(const Tables& T, const History& H, const Work& W, const Model&
         S = *sa;
         flags |= kHasTheta;
         ++nb;
-        const bool rows = !(sa->mode == kSearchSame && !sa->saveB2);
-        if (rows) { flags |= kHasRows; nb += T.N; }
+        flags |= kHasRows;
+        nb += T.N;
     }
     if (nb == 0) return;
     const int ept = (T.N + kStageThreads - 1) / kStageThreads;
