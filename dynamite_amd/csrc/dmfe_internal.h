@@ -116,7 +116,7 @@ inline int b2_taps_per_pass(int G, int m) {
     const int nt = (m + G - 1) / G;
     return nt < 1 ? 1 : (nt > 4 ? 4 : nt);
 }
-inline int rhs_default_G(int m) { return m <= 16 ? 8 : (m <= 32 ? 16 : 64); }
+inline int rhs_default_G(int m) { return m <= 16 ? 4 : (m <= 32 ? 16 : 64); }
 // column chunks per row: cw = ceil(N / split) entries per block
 inline int rhs_default_split(int N) { return N >= 256 ? (N / 256 > 1 ? N / 256 : 1) : 1; }
 inline int rhs_max_split(int N) { return 2 * rhs_default_split(N); }

@@ -374,12 +374,14 @@ constexpr int kRowThreads = 256;
 // row range.  The B2 work of a row falls steeply with theta_j (small theta_j: targets
 // spread over the whole history; theta_j near 1: mostly the last time interval), so a
 // contiguous row range per XCD (S = nb / 8) leaves some XCDs with most of the gathers.
-// S is the largest of 16, 8, 4, 2, 1 that divides nb / 8 (the identity if 8 does not
-// divide nb).
+// S is the largest power of two <= SMAX that divides nb / 8 (the identity if 8 does not
+// divide nb).  Measured at L = 512: SMAX = 32 for one instance (16 rows x 2 chunks per
+// stripe), 16 for the batch map (whose consecutive blocks are the instances of one row).
+template <int SMAX>
 __device__ inline int xcd_stripe(int b, int nb) {
     constexpr int X = 8;
     if (nb % X) return b;
-    int S = 16;
+    int S = SMAX;
     while (S > 1 && nb % (X * S)) S >>= 1;
     const int x = b % X, q = b / X;
     return ((q / S) * X + x) * S + q % S;
@@ -1187,7 +1189,7 @@ template <bool LOGR, int G, int NT>
 __global__ __launch_bounds__(kRowThreads) void rhs_rows_kernel(Tables T, History H, Work W, Model M, RhsArgs a) {
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
-    const int lb = xcd_stripe(blockIdx.x, gridDim.x);
+    const int lb = xcd_stripe<32>(blockIdx.x, gridDim.x);
     rhs_rows_body<LOGR, G, 0, NT>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red);
 }
 
@@ -1197,7 +1199,7 @@ __global__ __launch_bounds__(kRowThreads) void rhs_rows_tan_kernel(Tables T, His
                                                                    TanArgs X) {
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
-    const int lb = xcd_stripe(blockIdx.x, gridDim.x);
+    const int lb = xcd_stripe<32>(blockIdx.x, gridDim.x);
     rhs_rows_body<false, G, K>(T, H, W, M, a, lb / a.split, lb % a.split, dyn, red, &X);
 }
 
@@ -1792,7 +1794,7 @@ __global__ __launch_bounds__(kRowThreads) void batch_rhs_kernel(const InstDesc* 
                                                                 const int4* __restrict__ map, int n) {
     extern __shared__ double2 dyn[];
     __shared__ double red[kSums * (kRowThreads / kWave)];
-    const int4 mp = map[xcd_stripe(blockIdx.x, gridDim.x)];
+    const int4 mp = map[xcd_stripe<16>(blockIdx.x, gridDim.x)];
     if (mp.x < 0) return;
     const InstDesc& d = D[mp.x];
     const Ctl& c = *d.ctl;
@@ -1811,7 +1813,7 @@ __global__ __launch_bounds__(kRowThreads) void loop_rhs_kernel(InstDesc d, int n
     const int active = c.active, S = c.S;
     const RhsArgs a = step_rhs_args(c, d, n, LOGR);  // loaded with the control word
     if (!active || n >= S) return;
-    const int lb = xcd_stripe(blockIdx.x, gridDim.x);
+    const int lb = xcd_stripe<32>(blockIdx.x, gridDim.x);
     rhs_rows_body<LOGR, G, 0, NT>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red);
 }
 
@@ -1824,7 +1826,7 @@ __global__ __launch_bounds__(kRowThreads) void loop_rhs_tan_kernel(InstDesc d, i
     const int active = c.active, S = c.S;
     const RhsArgs a = step_rhs_args(c, d, n, 0);  // loaded with the control word
     if (!active || n >= S) return;
-    const int lb = xcd_stripe(blockIdx.x, gridDim.x);
+    const int lb = xcd_stripe<32>(blockIdx.x, gridDim.x);
     rhs_rows_body<false, G, K>(d.T, d.H, d.W, d.M, a, lb / d.split, lb % d.split, dyn, red, &d.X);
 }
 

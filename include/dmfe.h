@@ -314,7 +314,7 @@ int dmfe_get_rhs_shape(const dmfe_ctx* ctx, dmfe_rhs_shape* out);
 /* Sets the context's shape (before dmfe_tangent_init). */
 int dmfe_set_rhs_shape(dmfe_ctx* ctx, const dmfe_rhs_shape* shape);
 /* Process-wide shape table per grid class (N, m, log_response): contexts and batches take
- * their shape from it when created (default G = 8/16/64 by m, split = max(1, N/256)).
+ * their shape from it when created (default G = 4/16/64 by m, split = max(1, N/256)).
  * A batch runs one G for all instances: that of its first instance's class. */
 int dmfe_tuning_get(int64_t N, int m, int log_response, dmfe_rhs_shape* out);
 int dmfe_tuning_set(int64_t N, int m, int log_response, const dmfe_rhs_shape* shape);

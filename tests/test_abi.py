@@ -70,8 +70,8 @@ def test_driver_help_runs_without_a_device():
 def test_shape_table_validates_and_round_trips():
     """The RHS shape table needs no device: defaults by stencil width, validation of
     G against m and of split against N, and set/get of a tuned entry."""
-    assert dm.tuning_get(512, 10, 0) == (8, 2)
-    assert dm.tuning_get(128, 10, 0) == (8, 1)
+    assert dm.tuning_get(512, 10, 0) == (4, 2)
+    assert dm.tuning_get(128, 10, 0) == (4, 1)
     assert dm.tuning_get(512, 20, 0) == (16, 2)
     assert dm.tuning_get(512, 512, 0) == (64, 2)
     with pytest.raises(dm.DmfeError, match="G must be"):
@@ -82,5 +82,5 @@ def test_shape_table_validates_and_round_trips():
         dm.tuning_set(512, 10, 0, 4, 5)  # at most 2 * 512 / 256
     dm.tuning_set(300, 10, 1, 4, 2)
     assert dm.tuning_get(300, 10, 1) == (4, 2)
-    assert dm.tuning_get(300, 10, 0) == (8, 1)  # another class keeps the default
+    assert dm.tuning_get(300, 10, 0) == (4, 1)  # another class keeps the default
     dm.tuning_set(300, 10, 1, 8, 1)
