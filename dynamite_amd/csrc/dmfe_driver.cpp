@@ -31,6 +31,7 @@ struct Options {
     bool serk2 = false, rollback = false;  // the reference GPU loop's behaviours (off: CPU path)
     std::string out = "Results", grid_dir;
     long long save_every = 0, maint_every = 100000;
+    int sweeps = -1;  // -w, recorded in params.txt (the CPU build always runs one sweep)
 };
 
 void usage() {
@@ -79,10 +80,13 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "--fresh") o.resume = false;
         else if (a == "-S" || a == "--serk2") o.serk2 = std::string(val("-S")) != "false";
         else if (a == "--rollback") o.rollback = true;
-        else if (a == "-D" || a == "-g" || a == "-A" || a == "-w" || a == "-I") {
+        else if (a == "-w") {
+            // the sparsify sweep count: recorded, but the CPU build runs one sweep
+            // (simulation_runner.cpp:187-190), as here
+            o.sweeps = std::atoi(val("-w"));
+        } else if (a == "-D" || a == "-g" || a == "-A" || a == "-I") {
             // accepted for command-line compatibility (config.cpp:84-162); none of them changes
-            // this path: debug output, the CPU/GPU switch, asynchronous export, the sparsify
-            // sweep count (one sweep in the CPU build, simulation_runner.cpp:187-190) and the
+            // this path: debug output, the CPU/GPU switch, asynchronous export and the
             // version check of a resumed run
             const std::string v = val(a.c_str());
             std::fprintf(stderr, "note: option %s %s has no effect here\n", a.c_str(), v.c_str());
@@ -143,9 +147,47 @@ void write_params(const std::string& dir, const Options& o, dmfe_ctx* ctx, doubl
                  o.lambda, o.T0, o.Gamma);
     std::fprintf(f, "# Numerical Parameters\nlen = %lld\ntmax = %.16g\ndelta_t_min = %.16g\ndelta_max = %.16g\n", o.len,
                  o.tmax, o.dt_min, o.delta_max);
-    std::fprintf(f, "use_serk2 = %s\nlog_response_interp = %s\ntail_fit_enabled = false\nsparsify_sweeps = 1\n",
-                 o.serk2 ? "true" : "false", o.log_response ? "true" : "false");
+    std::fprintf(f, "use_serk2 = %s\nlog_response_interp = %s\ntail_fit_enabled = false\nsparsify_sweeps = %d\n",
+                 o.serk2 ? "true" : "false", o.log_response ? "true" : "false", o.sweeps);
     std::fprintf(f, "maxLoop = %lld\nrmax = [3, 13, 20, 80, 180, 320, 500, 720, 980, 1280, 1620]\n\n", o.max_loop);
+    // provenance of the grid package, mirrored from its grid_params.txt as the reference
+    // does (io_save_params.cpp:202-226): raw values, Tmax / alpha / delta as %.16g
+    std::fprintf(f, "# Grid Generation Parameters\n");
+    if (FILE* gp = std::fopen((o.grid_dir + "/grid_params.txt").c_str(), "r")) {
+        std::vector<std::pair<std::string, std::string>> kv;
+        char line[1024];
+        while (std::fgets(line, sizeof line, gp)) {
+            std::string l(line);
+            while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+            const size_t eq = l.find('=');
+            if (l.empty() || l[0] == '#' || eq == std::string::npos) continue;
+            kv.emplace_back(l.substr(0, eq), l.substr(eq + 1));
+        }
+        std::fclose(gp);
+        auto get = [&](const char* k) -> const std::string* {
+            for (auto& e : kv)
+                if (e.first == k) return &e.second;
+            return nullptr;
+        };
+        const char* raw[][2] = {{"subdir", "grid_subdir"}, {"len", "grid_len"}, {"Tmax", "grid_Tmax"},
+                                {"spline_order", "grid_spline_order"}, {"interp_method", "grid_interp_method"},
+                                {"interp_order", "grid_interp_order"}, {"fh_stencil", "grid_fh_stencil"},
+                                {"alpha", "grid_alpha"}, {"delta", "grid_delta"}};
+        for (auto& r : raw) {
+            const std::string* v = get(r[0]);
+            if (!v) continue;
+            const std::string k = r[0];
+            if (k == "Tmax" || k == "alpha" || k == "delta") {
+                char* end = nullptr;
+                const double d = std::strtod(v->c_str(), &end);
+                if (end != v->c_str()) { std::fprintf(f, "%s = %.16g\n", r[1], d); continue; }
+            }
+            std::fprintf(f, "%s = %s\n", r[1], v->c_str());
+        }
+        std::fprintf(f, "\n");
+    } else {
+        std::fprintf(f, "# grid_params.txt not found for len=%lld\n\n", o.len);
+    }
     std::fprintf(f, "# Current Simulation State\ncurrent_time = %.16g\ncurrent_loop = %lld\ncurrent_delta = %.16g\n",
                  t_last, (long long)loop, delta);
     // io_save_params.cpp:234: RK54, SSPRK104, or SERK2(q) with q = 2 (method - 2)

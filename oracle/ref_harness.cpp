@@ -29,6 +29,8 @@
 #include "grid/integration.hpp"
 #include "grid/pos_grid.hpp"
 #include "grid/interpolation_weights.hpp"
+#include "grid/grid_io.hpp"
+#include "io/io_utils.hpp"
 
 SimulationConfig config;
 SimulationData* sim = nullptr;
@@ -301,4 +303,35 @@ void ref_grid_weights(int which, int method, int n, int mfh, int64_t* inds, doub
 extern "C" void ref_grid_set_theta_ld(int64_t N, const long double* th) {
     g_theta.assign(th, th + N);
     generate_phi_grids(g_theta, g_phi1, g_phi2);
+}
+
+// ---- files written by the reference's own writers ------------------------------------
+// saveSimulationStateBinary (io_save_binary.cpp:22-116): data.bin version 1 of the
+// current state at `path` (and params.txt beside it, io_save_params.cpp), with the
+// control scalars set by ref_set_control.
+extern "C" void ref_save_binary(const char* path) {
+    saveSimulationStateBinary(std::string(path), config.delta, config.delta_t);
+}
+
+// The grid package of the generator state left by ref_grid_theta / ref_grid_int (plus
+// the given positions and stencil tables) under Grid_data/<subdir> of the working
+// directory: write_all_grids, the three interp-metadata writers and
+// write_grid_generation_params (grid_io.cpp:176-328).  inds/weights: A1, A2, B2 blocks.
+extern "C" void ref_grid_write(const char* subdir, const double* posA1, const double* posA2, const double* posB2,
+                               const int64_t* inds, const double* weights, int64_t m, double Tmax,
+                               int spline_order, const char* interp_method, int interp_order, int fh_stencil,
+                               double alpha, double delta) {
+    const size_t N = g_theta.size(), P = N * N;
+    std::vector<double> a(posA1, posA1 + P), b(posA2, posA2 + P), c(posB2, posB2 + P);
+    write_all_grids(g_theta, g_phi1, g_phi2, g_int, a, b, c, N, subdir);
+    for (int f = 0; f < 3; ++f) {
+        std::vector<int> it(P);
+        for (size_t k = 0; k < P; ++k) it[k] = (int)inds[f * P + k];
+        std::vector<double> w(weights + f * P * (size_t)m, weights + (f + 1) * P * (size_t)m);
+        if (f == 0) write_A1_interp_metadata(it, w, N, subdir);
+        else if (f == 1) write_A2_interp_metadata(it, w, N, subdir);
+        else write_B2_interp_metadata(it, w, N, subdir);
+    }
+    write_grid_generation_params(N, Tmax, spline_order, interp_method, interp_order, fh_stencil, subdir, "", alpha,
+                                 delta);
 }

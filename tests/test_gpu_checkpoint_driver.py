@@ -126,6 +126,16 @@ def test_driver_run_and_resume(tmp_path):
     txt = open(os.path.join(pdir, "params.txt")).read()
     for key in ("p = 3", "p2 = 4", "lambda = 0.5", "len = 64", "current_loop = 30", "current_method = RK54"):
         assert key in txt, key
+    # every key the reference's resume check parses (io_input.cpp:344-450), present in the
+    # reference's own params.txt (tests/golden/ref_state64, io_save_params.cpp) and here
+    keys = lambda t: {l.split("=", 1)[0].strip() for l in t.splitlines() if "=" in l and not l.startswith("#")}  # noqa: E731
+    resume = {"p", "p2", "lambda", "T0", "Gamma", "len", "delta_t_min", "delta_max", "use_serk2", "sparsify_sweeps",
+              "log_response_interp", "tail_fit_enabled"}
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_state64", "params.txt")
+    assert resume <= keys(open(gold).read())
+    assert resume <= keys(txt), resume - keys(txt)
+    # the grid package's provenance, mirrored as the reference does
+    assert "grid_len = 64" in txt and "grid_interp_method = poly" in txt, txt
     h, _ = parse_data_bin(os.path.join(pdir, "data.bin"))
     assert h["loop"] == 30
     # a second invocation resumes from data.bin and appends
