@@ -116,39 +116,113 @@ def gpu_run(g, prep, steps, warmup):
     return el, prof, state, control, T1, rhs_b2b_ms, tune, host
 
 
+def host_cpus():
+    """CPUs this process may use, the machine's logical CPUs and its physical cores
+    (distinct (package, core) pairs of /proc/cpuinfo among the allowed CPUs)."""
+    allowed = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    phys, cpu, pkg = set(), None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "processor":
+                    cpu, pkg = int(v), None
+                elif k == "physical id":
+                    pkg = v.strip()
+                elif k == "core id" and cpu in allowed:
+                    phys.add((pkg, v.strip()))
+    except OSError:
+        pass
+    return dict(nproc=os.cpu_count(), allowed=len(allowed), physical_cores=len(phys) or None)
+
+
+def _omp_threads(n):
+    """Sets the OpenMP thread count of the calling thread (libgomp, which the reference
+    build links) for the parallel regions it starts from now on."""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
 def cpu_baseline(g, state, control, budget_s=15.0):
     """The reference's own CPU step (built from its sources with its CPU flags) on a
-    bounded sample of the same workload - the same live state, the same dt; falls back
-    to the C oracle (one core) when the reference build is absent."""
+    bounded sample of the same workload - the same live state, the same dt - with one
+    OpenMP thread and with all physical cores (at most 16, the host share of one GPU),
+    plus the wall time of the CPU-sized configuration run to its end (L=128 quench,
+    Gamma=0.8, tmax 1e3) on both threads counts and on the GPU (dmfe_run).  Falls back to
+    the C oracle (one core) when the reference build is absent."""
     from oracle_lib import REF_FAST_PATH
-    cores = min(16, os.cpu_count() or 1)
+    cpus = host_cpus()
+    cores = max(1, min(16, cpus["physical_cores"] or cpus["allowed"]))
     kind = "reference"
     kw = dict(p=PARAMS["p"], p2=PARAMS["p2"], lam=PARAMS["lam"], T0=PARAMS["T0"], Gamma=PARAMS["Gamma"])
     if os.path.exists(REF_FAST_PATH):
-        os.environ["OMP_NUM_THREADS"] = str(cores)
         os.environ.setdefault("OMP_PROC_BIND", "close")
         from oracle_lib import RefLib
-        cpu = RefLib(g, path=REF_FAST_PATH, **kw)
+        make = lambda grid, **k: RefLib(grid, path=REF_FAST_PATH, **k)  # noqa: E731
     else:
         from oracle_lib import Oracle
-        cpu = Oracle(g, **kw)
+        make = Oracle
         kind, cores = "port", 1
     st = state
-    cpu.set_history(st["t1grid"], st["dtratio"], st["rvec"], st["drvec"], st["QK"], st["QR"], st["dQK"], st["dQR"])
-    cpu.interpolate(0)
-    cpu.set_control(control["delta_t"], control["delta"], control["delta_old"], int(control["loop"]),
-                    int(control["method"]))
-    n = 0
+
+    def sample(threads, budget):
+        if kind == "reference":
+            _omp_threads(threads)
+        cpu = make(g, **kw)
+        cpu.set_history(st["t1grid"], st["dtratio"], st["rvec"], st["drvec"], st["QK"], st["QR"], st["dQK"],
+                        st["dQR"])
+        cpu.interpolate(0)
+        cpu.set_control(control["delta_t"], control["delta"], control["delta_old"], int(control["loop"]),
+                        int(control["method"]))
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            cpu.loop_iteration()
+            n += 1
+            el = time.perf_counter() - t0
+            if el > budget or n >= 500:
+                return n, el
+    n1, el1 = sample(1, budget_s / 3)
+    n, el = sample(cores, budget_s)
+    log(f"[bench] CPU step at L=512: 1 thread {n1 / el1:.2f} steps/s, {cores} threads {n / el:.2f} steps/s")
+    # the CPU-sized configuration: L=128 quench (grid Tmax 1e4), Gamma = 0.8, tmax = 1e3
+    import dynamite_amd as dm
+    g128 = dm.generate_grid(128, 1e4, 5, dm.INTERP_POLY, 9, 0)
+    kw128 = dict(kw, Gamma=0.8)
+    full = {}
+    for threads in ((1, cores) if kind == "reference" else (1,)):
+        if kind == "reference":
+            _omp_threads(threads)
+        cpu = make(g128, **kw128)
+        cpu.init_quench()
+        t0 = time.perf_counter()
+        loops, t = 0, 0.0
+        while t < 1e3:
+            c = cpu.control()
+            if c["delta_t"] < 1e-5:  # the main loop's delta_t >= delta_t_min condition
+                break
+            t = cpu.loop_iteration()
+            loops += 1
+        full[f"threads_{threads}_s"] = time.perf_counter() - t0
+        full["loops"] = loops
+    prm = dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.8)
+    ctx = new_context(g128, prm, rows_hint=4096)
+    ctx.init_quench()
     t0 = time.perf_counter()
-    while True:
-        cpu.loop_iteration()
-        n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or n >= 500:
-            break
+    steps, _ = ctx.run(1e3, 10**9)
+    ctx.synchronize()
+    full["gpu_dmfe_run_s"] = time.perf_counter() - t0
+    full["gpu_loops"] = steps
+    ctx.close()
+    log(f"[bench] CPU-sized run (L=128 quench to its stop): {full}")
     return dict(value=n / el, unit="RK steps/s", cores=cores, kind=kind,
-                sample=f"{n} main-loop RK steps of the same L=512 live state (T1={len(st['t1grid'])}), "
-                       f"{el:.1f}s wall")
+                sample=f"{n} main-loop RK steps of the same L=512 live state (T1={len(st['t1grid'])}) on "
+                       f"{cores} OpenMP threads, {el:.1f}s wall; {n1} steps on 1 thread, {el1:.1f}s",
+                threads_1_value=n1 / el1, host=cpus,
+                cpu_sized_run=dict(workload="L=128 poly-9 grid (Tmax 1e4), p=3 q=4 lambda=0.5 Gamma=0.8 T0=inf, "
+                                            "quench to tmax 1e3 (the main loop stops earlier by delta_t < "
+                                            "delta_t_min)", **full))
 
 
 def kernel_microbench(g, reps=200):
@@ -212,11 +286,12 @@ def dense_microbench(N=512, order=9, B=4, reps=10):
                 fp64_mfma_peak_TFLOPs=78.6, dense_rhs_us=rhs_ms * 1e3)
 
 
-def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
+def sensitivity_run(tmax=1e4, h=1e-6, many_tmax=30.0):
     """Gradients of E and QK(t,0) at t_final w.r.t. (Gamma, lambda) for p=3+4, lambda=0.5,
     Gamma=0.7 on an L=256 poly-9 grid (forward-mode tangents carried with the run): the
-    cost against the same run without tangents, and, on a shorter horizon, central finite
-    differences (h = 1e-6) of runs replaying the nominal dt / integrator sequence."""
+    cost against the same run without tangents, and central finite differences (h = 1e-6)
+    at the same t_final of runs replaying the nominal dt / integrator sequence (recorded
+    from a host-driven nominal run, which makes the same steps as dmfe_run)."""
     import numpy as np
     import dynamite_amd as dm
     g = make_grid(256)
@@ -238,18 +313,19 @@ def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
     c.close()
     log(f"[bench] sensitivities: {n_grad} steps to t={t_final:.4g}: forward {el_fwd:.2f}s, with tangents "
         f"{el_grad:.2f}s ({el_grad / el_fwd:.2f}x); grad {grad}")
-    # finite-difference check on a shorter horizon with the dt sequence frozen
+    # finite differences at t_final with the dt sequence frozen: the nominal sequence
+    t0 = time.perf_counter()
     c = new_context(g, prm(), rows_hint=1024)
     c.init_quench()
-    c.tangent_init([1.0, 0.0], [0.0, 1.0])
     seq, t = [], 0.0
-    while t < fd_tmax:
+    while t < tmax:
         ctl = c.get_control()
         seq.append((ctl["delta_t"], ctl["method"]))
         t = c.loop_iteration()
-    dE, dQ = c.tangent_observables(t)
     c.close()
-    J = np.array([[dE[0], dE[1]], [dQ[0], dQ[1]]])
+    if len(seq) != n_grad or t != t_final:
+        raise RuntimeError(f"host-driven nominal run: {len(seq)} steps to t={t!r}, dmfe_run {n_grad} to {t_final!r}")
+    J = np.array([[grad[0], grad[1]], [grad[2], grad[3]]])
 
     def replay(G, lam):
         r = new_context(g, prm(G, lam), rows_hint=len(seq) + 64)
@@ -264,7 +340,9 @@ def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
     for col, (dg, dl) in enumerate(((h, 0.0), (0.0, h))):
         Jfd[:, col] = (replay(0.7 + dg, 0.5 + dl) - replay(0.7 - dg, 0.5 - dl)) / (2 * h)
     err = float(np.max(np.abs(J - Jfd)) / np.max(np.abs(Jfd)))
-    log(f"[bench] sensitivities vs finite differences at t={t:.4g} ({len(seq)} steps): max rel err {err:.2e}")
+    el_fd = time.perf_counter() - t0
+    log(f"[bench] sensitivities vs finite differences at t={t:.4g} ({len(seq)} steps, {el_fd:.0f}s): "
+        f"max rel err {err:.2e}")
     # 16 (Gamma, lambda) points (seeded), one after another and all at once
     rng = np.random.default_rng(BATCH_SEED + 16)
     pts = [(float(G), float(lam)) for G, lam in zip(rng.uniform(0.3, 1.0, 16), rng.uniform(0.0, 1.0, 16))]
@@ -279,18 +357,25 @@ def sensitivity_run(tmax=1e4, fd_tmax=1e2, h=1e-6, many_tmax=1e2):
     t0 = time.perf_counter()
     grads_m, tf_m, st_m = dm.gradients_many([g] * len(pl), pl, many_tmax, 10**12)
     el_many = time.perf_counter() - t0
-    same = bool(np.array_equal(np.array(seq_grads), grads_m))
+    seq_arr = np.array(seq_grads)
+    same = bool(np.array_equal(seq_arr, grads_m))
+    diff_pts = [i for i in range(len(pl)) if not np.array_equal(seq_arr[i], grads_m[i])]
+    max_rel = float(np.max(np.abs(seq_arr - grads_m)) / np.max(np.abs(seq_arr)))
+    if not same:
+        log(f"[bench] 16 points: concurrent differs from sequential at points {diff_pts}, max rel diff {max_rel:.2e}")
     log(f"[bench] 16 points to t={many_tmax:g} ({int(st_m.sum())} steps): one after another {el_seq:.2f}s, "
         f"concurrent {el_many:.2f}s ({el_seq / el_many:.2f}x), identical={same}")
     many = dict(points=len(pl), t_final=many_tmax, total_steps=int(st_m.sum()), sequential_s=el_seq,
-                concurrent_s=el_many, speedup=el_seq / el_many, identical=same,
+                concurrent_s=el_many, speedup=el_seq / el_many, identical=same, max_rel_diff=max_rel,
                 workload="Gamma ~ U[0.3,1], lambda ~ U[0,1] (seeded), same grid and model")
     return dict(workload=f"L=256 poly-9 grid (spline 5, Tmax 1e5), p=3+4, lambda=0.5, Gamma=0.7, quench to "
                          f"t_final={t_final:.4g} ({n_grad} adaptive RK steps)",
                 grad={"dE_dGamma": grad[0], "dE_dlambda": grad[1], "dQK0_dGamma": grad[2], "dQK0_dlambda": grad[3]},
                 mode="forward (tangent-linear), 2 directions, dt sequence frozen",
                 forward_s=el_fwd, with_tangents_s=el_grad, cost_ratio=el_grad / el_fwd,
-                fd_check={"t_final": t, "steps": len(seq), "h": h, "max_rel_err": err}, batched=many)
+                fd_check={"t_final": t, "steps": len(seq), "h": h, "max_rel_err": err,
+                          "jacobian_fd": Jfd.tolist(), "dt_sequence": "frozen (nominal run)"},
+                batched=many)
 
 
 BATCH_SEED = 20261203

@@ -421,7 +421,7 @@ int upload_grid(GridDev& gd, const dmfe_grid* g) {
     return DMFE_OK;
 }
 
-int alloc_work(Work& W, int N, int split) {
+int alloc_work(Work& W, int N, int split, hipStream_t stream) {
     const size_t bN = (size_t)N * sizeof(double), bP = (size_t)N * N * sizeof(double);
     HIP_TRY(hipMalloc(&W.posB1, 2 * bN));  // double-buffered (Work::posB1)
     HIP_TRY(hipMalloc(&W.posB2, 2 * bP));
@@ -435,7 +435,8 @@ int alloc_work(Work& W, int N, int split) {
     HIP_TRY(hipMalloc(&W.hK, bN * kMaxStages));
     HIP_TRY(hipMalloc(&W.hR, bN * kMaxStages));
     HIP_TRY(hipMalloc(&W.scal, 8 * sizeof(double)));
-    HIP_TRY(hipMemset(W.scal, 0, 8 * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(W.scal, 0, 8 * sizeof(double), stream));  // ordered with the batch's stream
+    HIP_TRY(hipStreamSynchronize(stream));
     return DMFE_OK;
 }
 
@@ -654,7 +655,7 @@ extern "C" int dmfe_batch_create(int n, const dmfe_grid* grids, const dmfe_param
         x.cap = cap;
         int rc = alloc_hist(x.H, cap, N);
         if (rc) return fail(rc);
-        rc = alloc_work(x.W, N, rhs_max_split(N));
+        rc = alloc_work(x.W, N, rhs_max_split(N), b->stream);
         if (rc) return fail(rc);
         maxm = std::max(maxm, (int)g->m);
         const int cw = (N + x.split - 1) / x.split;
@@ -1049,7 +1050,8 @@ extern "C" int dmfe_batch_record(dmfe_batch* b, int cap) {
     drop_graphs(b);
     if (cap <= 0) return DMFE_OK;
     HIP_TRY(hipMalloc(&b->d_obs, (size_t)n_inst(b) * cap * 4 * sizeof(double)));
-    HIP_TRY(hipMemset(b->d_obs, 0, (size_t)n_inst(b) * cap * 4 * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(b->d_obs, 0, (size_t)n_inst(b) * cap * 4 * sizeof(double), b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
     b->obs_cap = cap;
     b->obs_on = true;
     return DMFE_OK;

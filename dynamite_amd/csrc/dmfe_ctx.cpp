@@ -722,7 +722,10 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
          hipMemcpy(c->W.posB1 + N, ones.data(), bN, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(c->W.posB2, zeros.data(), bP, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(c->W.posB2 + P, zeros.data(), bP, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemset(c->W.scal, 0, 8 * sizeof(double)) == hipSuccess;
+         // on the context's own stream: a null-stream memset is not ordered with it and, with
+         // other host threads using the device, can land after the context's first kernels
+         hipMemsetAsync(c->W.scal, 0, 8 * sizeof(double), c->stream) == hipSuccess &&
+         hipStreamSynchronize(c->stream) == hipSuccess;
     if (!up) return fail(DMFE_ERR_HIP, "dmfe_ctx_create: upload failed");
     int rc = ensure_rows(c, std::max<int64_t>(rows_hint, 64));
     if (rc) { std::string msg = g_err; free_ctx(c); return set_error(rc, msg); }

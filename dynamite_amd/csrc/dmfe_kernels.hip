@@ -784,11 +784,14 @@ __device__ __forceinline__ void b2_lists(const Tables& T, const History& H, cons
 // B2 of the primal history and of K tangent histories in one pass (linear form): the
 // weights, positions and Hermite coefficients are computed once per tap and applied to
 // every history; the primal's operations are exactly those of b2_groups<false>.
+// Entries of the last time interval are summed tap by tap in one lane when the plain
+// kernel reads them from its LDS rows (tail_rows), with b2_lists' operation order, so
+// that the primal stays bit-identical to the run without tangents.
 template <int G, int K>
 __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& H, const TanArgs& X, const RhsArgs& a,
                                                 int j, int c0, int C, const int* b2ix, const int* b2s,
                                                 const double* b2u, const double* b2rr, double2* b2, double2* tb2,
-                                                int cw) {
+                                                int cw, bool tail_rows) {
     const int N = T.N, m = T.m, T1 = a.T1;
     constexpr int GPW = kWave / G;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
@@ -817,7 +820,17 @@ __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& 
         double ak[1 + K], ar[1 + K];
 #pragma unroll
         for (int h = 0; h <= K; ++h) ak[h] = ar[h] = 0.0;
-        if (ev) {
+        if (ev && tail_rows && !b2int) {
+            if (d0 == 0)
+                for (int dd = 0; dd < m; ++dd) {
+                    const double wd = wk[dd];
+#pragma unroll
+                    for (int h = 0; h <= K; ++h) {
+                        const double2 g1 = Dh[h][o1 + dd];
+                        b2_tap(c, wd, Vh[h][o0 + dd], Vh[h][o1 + dd], g1, g1, ak[h], ar[h]);
+                    }
+                }
+        } else if (ev) {
             for (int dd = d0; dd < m; dd += G) {
                 const double wd = wk[dd];
 #pragma unroll
@@ -1033,7 +1046,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
         // instruction touch 64/G entries x a few contiguous cache lines
         if constexpr (K == 0)
             b2_groups<LOGR, G, NT>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tail_rows ? rows : nullptr);
-        else b2_groups_multi<G, K>(T, H, *X, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tb2, cw);
+        else b2_groups_multi<G, K>(T, H, *X, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tb2, cw, tail_rows);
     }
     __syncthreads();
 

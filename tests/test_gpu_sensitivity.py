@@ -100,6 +100,23 @@ def test_tangent_run_leaves_primal_unchanged(nominal):
         assert np.array_equal(st[k], s2[k]), k
 
 
+def test_tangent_run_leaves_primal_unchanged_L256():
+    """The same on the sensitivity workload's grid (L=256 poly 9, Tmax 1e5) through dmfe_run,
+    far enough that B2 targets lie both inside the history and in its last interval."""
+    g = dm.generate_grid(256, 1e5, 5, dm.INTERP_POLY, 9, 0)
+    states = []
+    for tangents in (True, False):
+        ctx = dm.Context(g, params(), rows_hint=700)
+        ctx.init_quench()
+        if tangents:
+            ctx.tangent_init([1.0, 0.0], [0.0, 1.0])
+        ctx.run(float("inf"), 600)
+        states.append(ctx.get_state())
+        ctx.close()
+    for k in ("t1grid", "QK", "QR", "dQK", "dQR", "rvec", "drvec"):
+        assert np.array_equal(states[0][k], states[1][k]), k
+
+
 def test_matches_gpu_finite_differences(nominal):
     seq, t = nominal[0], nominal[1]
     J, Jfd = jac(nominal), fd(replay_gpu, seq, t)
@@ -164,3 +181,37 @@ def test_gradients_many_matches_single_points():
         c.close()
         assert np.array_equal(grads[i], g1), (i, grads[i], g1)
         assert tf[i] == t1 and st[i] == n1
+
+
+def test_tangents_match_reference_build_gradient_L256():
+    """The sensitivity workload's grid (L=256 poly 9, Tmax 1e5) and model (p=3+4, lambda
+    0.5, Gamma 0.7) from the quench to t ~ 100: forward-mode tangents of the GPU path,
+    replaying the reference's own step sequence, against the gradient the reference CPU
+    path gives by central finite differences (h = 1e-6) of runs of the reference sources
+    replaying that sequence (tests/golden/ref_gradient256.npz, written by
+    tests/golden/make_ref_gradient256.py from oracle/_ref).  Bars: the nominal energy and
+    QK(t,0) within 1e-10 relative, the Jacobian within 1e-6 of its largest entry."""
+    import os
+
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_gradient256.npz"))
+    Ng, Tg, spline, order, p, p2, lam, G, T0, h = f["params"]
+    g = dm.generate_grid(int(Ng), Tg, int(spline), dm.INTERP_POLY, int(order), 0)
+    ctx = dm.Context(g, dm.default_params(p=int(p), p2=int(p2), lambda_=lam, T0=T0, Gamma=G),
+                     rows_hint=len(f["dt"]) + 64)
+    ctx.init_quench()
+    ctx.tangent_init([1.0, 0.0], [0.0, 1.0])
+    print(f"L={int(Ng)}: replaying {len(f['dt'])} reference steps with tangents", flush=True)
+    for dt, method in zip(f["dt"], f["method"]):
+        ctx.set_control(float(dt), 1.0, 0.0, 10, int(method))
+        ctx.rk_step()
+    t = float(f["t_final"])
+    E, q0 = ctx.observables(t)
+    dE, dQ = ctx.tangent_observables(t)
+    ctx.close()
+    E0, Q0 = f["observables"]
+    assert abs(E - E0) <= 1e-10 * abs(E0) and abs(q0 - Q0) <= 1e-10 * abs(Q0), (E, E0, q0, Q0)
+    J = np.array([[dE[0], dE[1]], [dQ[0], dQ[1]]])
+    Jref = f["jacobian"]
+    err = np.abs(J - Jref).max() / np.abs(Jref).max()
+    print("J gpu", J.tolist(), "J reference FD", Jref.tolist(), f"max rel err {err:.2e}", flush=True)
+    assert err <= 1e-6
