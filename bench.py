@@ -116,6 +116,25 @@ def gpu_run(g, prep, steps, warmup):
     return el, prof, state, control, T1, rhs_b2b_ms, tune, host
 
 
+def rhs_only_run(g, prep, reps):
+    """The live L=512 state of the plain workload (`prep` dmfe_run steps from the quench),
+    then `reps` back-to-back RHS launches of the installed shape on it: the dispatches
+    that counter runs select by kernel name."""
+    import dynamite_amd as dm
+    prm = dm.default_params(p=PARAMS["p"], p2=PARAMS["p2"], lambda_=PARAMS["lam"], T0=PARAMS["T0"],
+                            Gamma=PARAMS["Gamma"])
+    ctx = new_context(g, prm, rows_hint=prep + 128)
+    ctx.init_quench()
+    n, _ = ctx.run(1e300, prep)
+    ctx.synchronize()
+    ms = dm.time_rhs(ctx, reps)
+    out = dict(prep_steps=n, T1=ctx.T1, reps=reps, rhs_us=1e3 * ms,
+               achieved_GBs=algorithmic_bytes_rhs(g.N, g.m) / (ms * 1e-3) / 1e9)
+    ctx.close()
+    log(f"[bench] rhs only: {out}")
+    return out
+
+
 def host_cpus():
     """CPUs this process may use, the machine's logical CPUs and its physical cores
     (distinct (package, core) pairs of /proc/cpuinfo among the allowed CPUs)."""
@@ -506,6 +525,9 @@ def main():
                     help="issue the step launches directly instead of replaying hipGraphs (same kernels)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain HBM copy of this many MiB (counter calibration) and exit")
+    ap.add_argument("--rhs-only", type=int, default=0,
+                    help="only prepare the live L=512 state (--prep steps) and launch this many RHS "
+                         "evaluations back to back on it (for counter runs), then exit")
     a = ap.parse_args()
     global USE_GRAPHS
     USE_GRAPHS = not a.no_graphs
@@ -530,6 +552,9 @@ def main():
         return
     N = 512
     g = make_grid(N)
+    if a.rhs_only > 0:
+        print(json.dumps({"rhs_only": rhs_only_run(g, a.prep, a.rhs_only)}), flush=True)
+        return
     log(f"[bench] GPU: {a.prep} prep + {a.warmup} warmup + {a.steps} timed steps")
     el, prof, state, control, T1, rhs_b2b_ms, tune, host = gpu_run(g, a.prep, a.steps, a.warmup)
     ms_step = 1e3 * el / a.steps

@@ -4,7 +4,11 @@
         per-kernel dispatch count and duration statistics (kernel trace), optionally
         over the last `last_k` dispatches of each kernel;
     python -m dynamite_amd.rocprof_summary pmc <dir> [last_k]
-        mean of every collected counter per kernel over its last `last_k` dispatches.
+        mean of every collected counter per kernel over its last `last_k` dispatches;
+    python -m dynamite_amd.rocprof_summary steps <dir> [last_k]
+        the device loop's steps (dispatches from one loop_begin_kernel to the next) over the
+        last `last_k` complete steps: wall time per step, kernel-busy time per kernel, and
+        the idle time between dispatches.
 <dir> is the rocprofv3 -d directory (searched recursively for *.db).
 """
 import glob
@@ -49,7 +53,33 @@ def pmc(path, k):
         print(f"{_short(name)[:58]:58s} {c:14s} dispatches={len(vs):6d} last={len(last):5d} mean={st.mean(last):14.1f}")
 
 
+def steps(path, k):
+    con = _db(path)
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    nm = "name" if "name" in cols else "kernel_name"
+    rows = [(_short(n), a, b) for n, a, b in con.execute(f"select {nm}, start, end from kernels order by start")]
+    rows = [r for r in rows if r[0].startswith("loop_")]
+    heads = [i for i, r in enumerate(rows) if r[0] == "loop_begin_kernel"]
+    # steps in which the instance ran (a launch after the loop stopped exits at once)
+    spans = [(a, b) for a, b in zip(heads[:-1], heads[1:])
+             if max(r[2] - r[1] for r in rows[a:b] if r[0].startswith("loop_rhs")) > 4000][-k:]
+    if not spans:
+        raise SystemExit("no complete device-loop steps in the trace")
+    wall, busy, per = [], [], {}
+    for a, b in spans:
+        wall.append((rows[b][1] - rows[a][1]) / 1e3)
+        busy.append(sum(r[2] - r[1] for r in rows[a:b]) / 1e3)
+        for r in rows[a:b]:
+            per.setdefault(r[0], []).append((r[2] - r[1]) / 1e3)
+    n = len(spans)
+    print(f"steps={n}  wall/step={st.mean(wall):.2f} us  kernel-busy/step={st.mean(busy):.2f} us  "
+          f"idle between dispatches/step={st.mean(wall) - st.mean(busy):.2f} us")
+    print(f"{'kernel':58s} {'per_step':>8s} {'avg_us':>8s} {'us/step':>8s}")
+    for name, ds in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        print(f"{name[:58]:58s} {len(ds) / n:8.2f} {st.mean(ds):8.2f} {sum(ds) / n:8.2f}")
+
+
 if __name__ == "__main__":
     mode, path = sys.argv[1], sys.argv[2]
     k = int(sys.argv[3]) if len(sys.argv) > 3 else 10**9
-    {"trace": trace, "pmc": pmc}[mode](path, k)
+    {"trace": trace, "pmc": pmc, "steps": steps}[mode](path, k)
