@@ -6,6 +6,7 @@ Compiles in-tree with hipcc for gfx950 only.  Objects go to dynamite_amd/_build/
 the shared library to dynamite_amd/libdmfe.so, the command-line driver (g++ over the C
 ABI) to dynamite_amd/dmfe_run.
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -23,7 +24,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{os.p
 # packed FP32 instructions stay disabled in device code
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
-SOURCES = ["dmfe_kernels.hip", "dmfe_ctx.cpp", "dmfe_grid.cpp", "dmfe_batch.cpp"]
+SOURCES = ["dmfe_rhs.hip", "dmfe_loop.hip", "dmfe_kernels.hip", "dmfe_ctx.cpp", "dmfe_grid.cpp", "dmfe_batch.cpp"]
 
 
 def _run(cmd):
@@ -38,7 +39,7 @@ def build(verbose=False):
     if not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
@@ -49,8 +50,13 @@ def build(verbose=False):
             cmd = [HIPCC, *COMMON, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", "-o", obj, path]
         if verbose:
             print(" ".join(cmd), flush=True)
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    # the translation units compile independently (the RHS instantiations dominate)
+    workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     shutil.move(tmp, LIB)

@@ -62,7 +62,7 @@ def steps(path, k):
     heads = [i for i, r in enumerate(rows) if r[0] == "loop_begin_kernel"]
     # steps in which the instance ran (a launch after the loop stopped exits at once)
     spans = [(a, b) for a, b in zip(heads[:-1], heads[1:])
-             if max(r[2] - r[1] for r in rows[a:b] if r[0].startswith("loop_rhs")) > 4000][-k:]
+             if max((r[2] - r[1] for r in rows[a:b] if r[0].startswith(("loop_rhs", "loop_fused"))), default=0) > 4000][-k:]
     if not spans:
         raise SystemExit("no complete device-loop steps in the trace")
     wall, busy, per = [], [], {}
