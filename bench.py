@@ -135,6 +135,56 @@ def rhs_only_run(g, prep, reps):
     return out
 
 
+def full_run(g, tmax, segment=10000, wall_limit=900.0):
+    """The plain L=512 configuration integrated from the quench towards t = tmax with
+    dmfe_run (device-side step control, the every-1e5-loops maintenance included), in
+    segments of `segment` loops: per segment the loop count, t, dt, integrator, T1, E(t)
+    and QK(t,0); T1 before and after each maintenance.  Stops at tmax, when the main loop
+    stops by itself (delta_t < delta_t_min) or after wall_limit seconds."""
+    import dynamite_amd as dm
+    prm = dm.default_params(p=PARAMS["p"], p2=PARAMS["p2"], lambda_=PARAMS["lam"], T0=PARAMS["T0"],
+                            Gamma=PARAMS["Gamma"])
+    ctx = new_context(g, prm, rows_hint=segment + 256)
+    ctx.init_quench()
+    recs, maint = [], []
+    t0 = time.perf_counter()
+    loop, t, stop = 0, 0.0, "tmax"
+    while True:
+        T1_before = ctx.T1
+        # the correlation records of the segment (t, E(t), QK(t,0) per loop; the last one
+        # is the segment's end)
+        n, corr = ctx.run(tmax, loop + segment, corr_cap=segment)
+        ctx.synchronize()
+        c = ctx.get_control()
+        loop = int(c["loop"])
+        if n == 0:
+            stop = "delta_t < delta_t_min" if c["delta_t"] < 1e-5 else "loop stopped"
+            break
+        t, E, qk0 = (float(v) for v in corr[-1])
+        el = time.perf_counter() - t0
+        rec = dict(loop=loop, t=t, dt=c["delta_t"], method="RK54" if c["method"] == 1 else "SSPRK104",
+                   T1=ctx.T1, E=E, QK_t0=qk0, wall_s=el)
+        recs.append(rec)
+        # the maintenance ran if a multiple of 1e5 loops was crossed in this segment
+        if loop // 100000 > (loop - n) // 100000:
+            maint.append(dict(loop=(loop // 100000) * 100000, T1_segment_start=T1_before, T1_segment_end=ctx.T1))
+        log(f"[bench] full run: loop {loop} t {t:.6g} dt {c['delta_t']:.4g} {rec['method']} T1 {ctx.T1} "
+            f"E {E:.10g} QK(t,0) {qk0:.10g} ({el:.0f} s)")
+        if t >= tmax:
+            break
+        if n < segment:
+            stop = "delta_t < delta_t_min" if c["delta_t"] < 1e-5 else "loop stopped"
+            break
+        if el > wall_limit:
+            stop = "wall limit"
+            break
+    ctx.close()
+    return dict(workload="plain quench, L=512 poly-9 grid (spline 5, Tmax 1e5), p=3 q=4 lambda=0.5 Gamma=0.6 "
+                         "T0=inf, dmfe_run from the quench", tmax=tmax, stop=stop, loops=loop, t_final=t,
+                wall_s=time.perf_counter() - t0, E_final=recs[-1]["E"], QK_t0_final=recs[-1]["QK_t0"],
+                maintenance=maint, segments=recs)
+
+
 def host_cpus():
     """CPUs this process may use, the machine's logical CPUs and its physical cores
     (distinct (package, core) pairs of /proc/cpuinfo among the allowed CPUs)."""
@@ -528,6 +578,9 @@ def main():
     ap.add_argument("--rhs-only", type=int, default=0,
                     help="only prepare the live L=512 state (--prep steps) and launch this many RHS "
                          "evaluations back to back on it (for counter runs), then exit")
+    ap.add_argument("--full-run", type=float, default=0.0,
+                    help="only integrate the L=512 configuration from the quench to this t (1e5 for the "
+                         "production length) with dmfe_run, reporting every 10000 loops, then exit")
     a = ap.parse_args()
     global USE_GRAPHS
     USE_GRAPHS = not a.no_graphs
@@ -554,6 +607,9 @@ def main():
     g = make_grid(N)
     if a.rhs_only > 0:
         print(json.dumps({"rhs_only": rhs_only_run(g, a.prep, a.rhs_only)}), flush=True)
+        return
+    if a.full_run > 0:
+        print(json.dumps({"full_run": full_run(g, a.full_run)}), flush=True)
         return
     log(f"[bench] GPU: {a.prep} prep + {a.warmup} warmup + {a.steps} timed steps")
     el, prof, state, control, T1, rhs_b2b_ms, tune, host = gpu_run(g, a.prep, a.steps, a.warmup)

@@ -412,61 +412,96 @@ void grid_bspline_weights(const std::vector<long double>& x, const std::vector<l
     for (auto& x_ : th) x_.join();
 }
 
-// ---- pos grids (theta -> fractional 1-based index), written for format parity ---------
+// ---- pos grids (theta value -> fractional 1-based index), written for format parity ----
+// The theta map x -> theta(x) of theta_values is smooth and increasing in the continuous
+// index x in [0, N-1].  Its inverse at a value v: the map is tabulated at kSub points per
+// index interval, v is located in the table, and the bracketing table interval is shrunk
+// by safeguarded secant steps on the map itself (regula falsi with the Illinois
+// modification, all queries evaluated together per step) until the bracket is below
+// kTol index units or kIter steps have run.  Values at or beyond the ends of the map give
+// positions 1 and N.  The result is the position y = x + 1 with theta(x) = v to the
+// accuracy of the map's evaluation.
+namespace {
+void invert_theta_map(std::size_t N, double Tmax, double alpha, double delta, const std::vector<double>& tab_x,
+                      const std::vector<ld>& tab, const std::vector<ld>& q, double* out) {
+    constexpr int kIter = 8;
+    constexpr double kTol = 1e-13;
+    const std::size_t Q = q.size();
+    std::vector<double> a(Q), b(Q), x(Q);
+    std::vector<ld> fa(Q), fb(Q);
+    std::vector<signed char> side(Q, 0), done(Q, 0);
+    for (std::size_t k = 0; k < Q; ++k) {
+        const ld v = q[k];
+        if (!(v > tab.front())) { out[k] = 1.0; done[k] = 1; continue; }
+        if (!(v < tab.back())) { out[k] = (double)N; done[k] = 1; continue; }
+        const std::size_t i = (std::size_t)(std::upper_bound(tab.begin(), tab.end(), v) - tab.begin());  // tab[i-1] <= v < tab[i]
+        a[k] = tab_x[i - 1]; fa[k] = tab[i - 1] - v;
+        b[k] = tab_x[i];     fb[k] = tab[i] - v;
+        if (fa[k] == 0) { out[k] = a[k] + 1.0; done[k] = 1; }
+    }
+    std::vector<double> xs;
+    std::vector<std::size_t> who;
+    std::vector<ld> fx;
+    for (int it = 0; it < kIter; ++it) {
+        xs.clear();
+        who.clear();
+        for (std::size_t k = 0; k < Q; ++k) {
+            if (done[k]) continue;
+            double c = (double)(((ld)a[k] * fb[k] - (ld)b[k] * fa[k]) / (fb[k] - fa[k]));
+            if (!(c > a[k] && c < b[k])) c = 0.5 * (a[k] + b[k]);
+            x[k] = c;
+            xs.push_back(c);
+            who.push_back(k);
+        }
+        if (who.empty()) break;
+        theta_values(xs, N, Tmax, alpha, delta, fx);
+        for (std::size_t r = 0; r < who.size(); ++r) {
+            const std::size_t k = who[r];
+            const ld f = fx[r] - q[k];
+            if (f == 0) { a[k] = b[k] = x[k]; fa[k] = fb[k] = 0; }
+            else if (f < 0) {
+                a[k] = x[k]; fa[k] = f;
+                if (side[k] < 0) fb[k] *= 0.5L;  // Illinois: the retained end keeps drawing the secant
+                side[k] = -1;
+            } else {
+                b[k] = x[k]; fb[k] = f;
+                if (side[k] > 0) fa[k] *= 0.5L;
+                side[k] = 1;
+            }
+            if (b[k] - a[k] <= kTol) done[k] = 2;
+        }
+    }
+    for (std::size_t k = 0; k < Q; ++k) {
+        if (done[k] == 1) continue;
+        const double c = (fb[k] == fa[k]) ? a[k] : (double)(((ld)a[k] * fb[k] - (ld)b[k] * fa[k]) / (fb[k] - fa[k]));
+        out[k] = std::min(std::max(c, a[k]), b[k]) + 1.0;
+    }
+}
+}  // namespace
+
 void grid_pos(std::size_t N, double Tmax, double alpha, double delta, const std::vector<long double>& theta,
               const std::vector<long double>& phi1, const std::vector<long double>& phi2,
               double* posA1, double* posA2, double* posB2) {
-    std::size_t over = 20;
-    if (const char* e = std::getenv("DMFE_POS_OVERSAMPLE")) { long v = std::strtol(e, nullptr, 10); if (v > 1) over = (std::size_t)v; }
-    int mwin = 12;
-    if (const char* e = std::getenv("DMFE_POS_BARY_M")) mwin = std::max(2, std::min(32, (int)std::strtol(e, nullptr, 10)));
-    const std::size_t M = N * over;
-    std::vector<double> idx(M);
-    std::vector<ld> ys(M), xs;
-    for (std::size_t k = 0; k < M; ++k) {
-        ys[k] = 1.0L + ((ld)N - 1.0L) * ((ld)k / (ld)(M - 1));
-        idx[k] = (double)(ys[k] - 1.0L);
-    }
-    theta_values(idx, N, Tmax, alpha, delta, xs);
-    for (std::size_t k = 1; k < M; ++k)
-        if (xs[k] <= xs[k - 1]) xs[k] = std::nextafter(xs[k - 1], std::numeric_limits<ld>::infinity());
-    int m = std::max(4, std::min(16, mwin));
-    if (m > (int)M) m = (int)M;
-    const int nLeft = std::max(0, (int)M - m) + 1;
-    std::vector<ld> bw((std::size_t)nLeft * m);
-    for (int L = 0; L < nLeft; ++L)
-        for (int j = 0; j < m; ++j) {
-            ld prod = 1;
-            for (int k = 0; k < m; ++k)
-                if (k != j) prod *= xs[L + j] - xs[L + k];
-            bw[(std::size_t)L * m + j] = 1.0L / prod;
-        }
-    auto eval = [&](ld v) -> ld {
-        if (v <= xs.front()) return ys.front();
-        if (v >= xs.back()) return ys.back();
-        std::size_t i = (std::size_t)(std::upper_bound(xs.begin(), xs.end(), v) - xs.begin());
-        i = std::max<std::size_t>(1, i) - 1;
-        const int L = std::min(std::max((int)i - m / 2 + 1, 0), nLeft - 1);
-        for (int j = 0; j < m; ++j)
-            if (v == xs[L + j]) return ys[L + j];
-        ld num = 0, den = 0;
-        for (int j = 0; j < m; ++j) {
-            const ld w = bw[(std::size_t)L * m + j] / (v - xs[L + j]);
-            den += w;
-            num += w * ys[L + j];
-        }
-        return num / den;
-    };
-    const ld tiny = 1e-200L;
+    constexpr std::size_t kSub = 8;  // table points per index interval
+    const std::size_t M = (N - 1) * kSub + 1;
+    std::vector<double> tab_x(M);
+    for (std::size_t k = 0; k < M; ++k) tab_x[k] = (double)k / (double)kSub;
+    std::vector<ld> tab;
+    theta_values(tab_x, N, Tmax, alpha, delta, tab);
+    for (std::size_t k = 1; k < M; ++k)  // the clamp at theta = 0 and 1 leaves flat ends: keep them ordered
+        if (tab[k] < tab[k - 1]) tab[k] = tab[k - 1];
+    const std::size_t P = N * N;
+    std::vector<ld> qB2(P);
+    const ld tiny = 1e-200L;  // the B2 query point theta_i / (phi2 - 1e-200), as in the weight tables
     for (std::size_t i = 0; i < N; ++i)
         for (std::size_t j = 0; j < N; ++j) {
             const std::size_t k = i * N + j;
-            posA1[k] = (double)eval(phi1[k]);
-            posA2[k] = (double)eval(phi2[k]);
             const ld den = phi2[k] - tiny;
-            const ld arg = (std::fabs(den) < 1e-300L) ? ((theta[i] > 0) ? xs.back() : xs.front()) : theta[i] / den;
-            posB2[k] = (double)eval(arg);
+            qB2[k] = (std::fabs(den) < 1e-300L) ? ((theta[i] > 0) ? tab.back() : tab.front()) : theta[i] / den;
         }
+    invert_theta_map(N, Tmax, alpha, delta, tab_x, tab, phi1, posA1);
+    invert_theta_map(N, Tmax, alpha, delta, tab_x, tab, phi2, posA2);
+    invert_theta_map(N, Tmax, alpha, delta, tab_x, tab, qB2, posB2);
 }
 
 }  // namespace dmfe

@@ -7,6 +7,7 @@
 // Nothing here re-implements reference arithmetic: every numerical call goes to
 // the reference's own functions (interpolate, update, QKstep, ... and the grid
 // generator pieces).  This file is test infrastructure only.
+#include <algorithm>
 #include <vector>
 #include <string>
 #include <cstring>
@@ -118,6 +119,10 @@ void ref_set_history(int64_t T1, const double* t1grid, const double* dtratio,
 }
 
 int64_t ref_T1(void) { return (int64_t)sim->h_t1grid.size(); }
+
+// QK(t,0) of the last history row, and the time grid alone (cheap accessors for long runs)
+double ref_last_qk0(void) { return sim->h_QKv[(sim->h_t1grid.size() - 1) * config.len]; }
+void ref_get_t1grid(double* out) { std::copy(sim->h_t1grid.begin(), sim->h_t1grid.end(), out); }
 
 void ref_get_history(double* t1grid, double* dtratio, double* rvec, double* drvec,
                      double* QK, double* QR, double* dQK, double* dQR) {

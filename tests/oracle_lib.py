@@ -222,6 +222,17 @@ class RefLib(_Common):
     def T1(self):
         return self.L.ref_T1()
 
+    def last_qk0(self):
+        """QK(t,0) of the last history row."""
+        self.L.ref_last_qk0.restype = C.c_double
+        return self.L.ref_last_qk0()
+
+    def t1grid(self):
+        out = np.zeros(self.T1)
+        self.L.ref_get_t1grid.argtypes = [_dp]
+        self.L.ref_get_t1grid(_p(out))
+        return out
+
     def init_quench(self):
         self.L.ref_init_quench()
 

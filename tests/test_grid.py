@@ -163,3 +163,29 @@ def test_generator_bit_exact_vs_reference(N, Tmax, method, order, fh, s):
                              inds.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p))
         assert np.array_equal(inds, g.inds[which])
         assert np.array_equal(w, g.weights[which])
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("N,Tmax", [(48, 1e3), (128, 1e4)])
+def test_pos_grids_agree_with_reference(N, Tmax):
+    """posA1/posA2/posB2 (theta value -> fractional 1-based index) are the inverse of the
+    theta map, computed here by root finding on the map; the reference generator
+    (pos_grid.cpp) interpolates an oversampled table barycentrically.  Both invert the
+    same map, so they agree to the reference's interpolation accuracy; the positions are
+    monotone in the query value and lie in [1, N]."""
+    ref = C.CDLL(REF_PATH)
+    g = dm.generate_grid(N, Tmax, 5, dm.INTERP_POLY, 9, 0, with_pos=True)
+    thl = (C.c_longdouble * N)()
+    assert dm.lib().dmfe_grid_theta_ld(C.c_int64(N), C.c_double(Tmax), C.c_double(0), C.c_double(0), thl) == 0
+    ref.ref_grid_set_theta_ld(C.c_int64(N), thl)
+    P = N * N
+    r = [np.zeros(P) for _ in range(3)]
+    ref.ref_grid_pos(C.c_double(Tmax), C.c_double(0), C.c_double(0), *[a.ctypes.data_as(C.c_void_p) for a in r])
+    for f in range(3):
+        mine = np.asarray(g.pos[f]).ravel()
+        assert mine.min() >= 1.0 and mine.max() <= N
+        assert np.abs(mine - r[f]).max() < 1e-6, (f, np.abs(mine - r[f]).max())
+    # monotone in the query: phi1 and the positions sorted together
+    q = np.asarray(g.phi1).ravel()
+    o = np.argsort(q, kind="stable")
+    assert np.all(np.diff(np.asarray(g.pos[0]).ravel()[o]) >= 0)
