@@ -578,6 +578,7 @@ def main():
     ap.add_argument("--rhs-only", type=int, default=0,
                     help="only prepare the live L=512 state (--prep steps) and launch this many RHS "
                          "evaluations back to back on it (for counter runs), then exit")
+    ap.add_argument("--dense-only", action="store_true", help="only the dense B-spline measurement")
     ap.add_argument("--full-run", type=float, default=0.0,
                     help="only integrate the L=512 configuration from the quench to this t (1e5 for the "
                          "production length) with dmfe_run, reporting every 10000 loops, then exit")
@@ -598,6 +599,9 @@ def main():
         sys.exit(2)
     if a.sensitivity_only:
         print(json.dumps({"sensitivities": sensitivity_run()}), flush=True)
+        return
+    if a.dense_only:
+        print(json.dumps({"dense_bspline": dense_microbench()}), flush=True)
         return
     if a.batch_only:
         out = batch_run(a.batch_prep, a.batch_steps, a.looped_steps)

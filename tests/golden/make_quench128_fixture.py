@@ -11,7 +11,12 @@ oracle/_ref/libdmfe_ref_strict.so (the loop of simulation_runner.cpp:125-304 thr
 oracle/ref_harness.cpp, observables as in simulation_runner.cpp:327-341).  Run from the
 repository root after `make -C oracle ref`:
 
-    python3 tests/golden/make_quench128_fixture.py
+    python3 tests/golden/make_quench128_fixture.py            # one OpenMP thread
+    python3 tests/golden/make_quench128_fixture.py 8          # the same run on 8 threads
+
+The 8-thread run (quench128_main_loop_8threads.npz) is the same reference build with its
+OpenMP reductions split over 8 threads: it measures how far the reference's own result
+moves under another summation order.
 """
 import os
 import sys
@@ -29,13 +34,15 @@ P, P2, LAM, GAMMA, T0 = 3, 4, 0.5, 0.8, 1e50
 DT_MIN, DELTA_MAX = 1e-5, 1e-10
 T_FINAL = 1e3
 EVERY = 25
-OUT = os.path.join(HERE, "quench128_main_loop.npz")
+THREADS = int(sys.argv[1]) if len(sys.argv) > 1 else 1
+OUT = os.path.join(HERE, "quench128_main_loop.npz" if THREADS == 1 else f"quench128_main_loop_{THREADS}threads.npz")
 
 
 def main():
-    # one thread: the reference's OpenMP reductions combine partial sums in thread arrival
-    # order, so more threads make the last bits (and, near the stall, the trajectory) vary
-    os.environ["OMP_NUM_THREADS"] = "1"
+    # one thread by default: the reference's OpenMP reductions combine partial sums in
+    # thread arrival order, so more threads make the last bits (and, near the stall, the
+    # trajectory) vary
+    os.environ["OMP_NUM_THREADS"] = str(THREADS)
     import dynamite_amd as dm
     from oracle_lib import RefLib, ref_available
 

@@ -144,11 +144,17 @@ def test_quench_free_running(N, Tmax, steps, log_resp):
     so = orc.state()
     qo = so["QK"][(len(so["t1grid"]) - 1) * N]
     assert same_dt == steps, f"dt sequences diverged ({same_dt}/{steps} equal)"
-    assert abs(eg - eo) <= 1e-9 * abs(eo)
-    assert abs(qg - qo) <= 1e-9 * abs(qo)
     sg = ctx.get_state()
-    rel_close(sg["QK"], so["QK"], 1e-9)
-    rel_close(sg["QR"], so["QR"], 1e-9)
+    errs = dict(E=abs(eg - eo) / abs(eo), QK_t0=abs(qg - qo) / abs(qo),
+                QK=np.abs(sg["QK"] - so["QK"]).max() / np.abs(so["QK"]).max(),
+                QR=np.abs(sg["QR"] - so["QR"]).max() / np.abs(so["QR"]).max())
+    print(f"N={N} Tmax={Tmax:g} {steps} steps log_response={log_resp}: " +
+          ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    # QK/QR per step within 1e-10 relative of the CPU reference path (the log-response
+    # form, QR interpolated as exp of a stencil of log QR, amplifies rounding by 1/QR: 1e-9)
+    bar = 1e-9 if log_resp else 1e-10
+    for k, v in errs.items():
+        assert v <= bar, (k, v)
 
 
 def test_sparsify_matches_oracle():
