@@ -434,8 +434,26 @@ def sensitivity_run(tmax=1e4, h=1e-6, many_tmax=30.0):
         log(f"[bench] 16 points: concurrent differs from sequential at points {diff_pts}, max rel diff {max_rel:.2e}")
     log(f"[bench] 16 points to t={many_tmax:g} ({int(st_m.sum())} steps): one after another {el_seq:.2f}s, "
         f"concurrent {el_many:.2f}s ({el_seq / el_many:.2f}x), identical={same}")
+    # the 16 points as one batch: the tangents ride inside the batch's stage and RHS
+    # launches (dmfe_batch_tangent_init), every instance with its own step control
+    b = dm.Batch([g] * len(pl), pl, rows_cap=int(st_m.max()) + 64)
+    t0 = time.perf_counter()
+    b.init_quench()
+    b.tangent_init([1.0, 0.0], [0.0, 1.0])
+    b.set_limits(many_tmax, 10**12)
+    nb = b.run(10**9, use_graph=True, chunk=16)
+    bE, bQ = b.tangent_observables()
+    el_batch = time.perf_counter() - t0
+    b.close()
+    grads_b = np.stack([bE[:, 0], bE[:, 1], bQ[:, 0], bQ[:, 1]], axis=1)
+    same_b = bool(np.array_equal(seq_arr, grads_b))
+    rel_b = float(np.max(np.abs(seq_arr - grads_b)) / np.max(np.abs(seq_arr)))
+    log(f"[bench] 16 points in one batch: {el_batch:.2f}s ({nb} batch steps, {el_seq / el_batch:.2f}x one after "
+        f"another), identical to the single-point gradients={same_b} (max rel diff {rel_b:.2e})")
     many = dict(points=len(pl), t_final=many_tmax, total_steps=int(st_m.sum()), sequential_s=el_seq,
                 concurrent_s=el_many, speedup=el_seq / el_many, identical=same, max_rel_diff=max_rel,
+                batch_s=el_batch, batch_steps=int(nb), batch_speedup=el_seq / el_batch, batch_identical=same_b,
+                batch_max_rel_diff=rel_b,
                 workload="Gamma ~ U[0.3,1], lambda ~ U[0,1] (seeded), same grid and model")
     return dict(workload=f"L=256 poly-9 grid (spline 5, Tmax 1e5), p=3+4, lambda=0.5, Gamma=0.7, quench to "
                          f"t_final={t_final:.4g} ({n_grad} adaptive RK steps)",

@@ -505,9 +505,31 @@ class Batch:
 
     def init_quench(self):
         _check(lib().dmfe_batch_init_quench(self._h), "dmfe_batch_init_quench")
+        self._tanK = 0
 
     def set_limits(self, tmax=float("inf"), max_loop=2**62):
         _check(lib().dmfe_batch_set_limits(self._h, C.c_double(tmax), C.c_int64(max_loop)), "dmfe_batch_set_limits")
+
+    def tangent_init(self, dGamma, dLambda):
+        """Forward-mode sensitivities of every instance along K = len(dGamma) directions
+        (dmfe_batch_tangent_init); right after init_quench."""
+        dG = np.ascontiguousarray(dGamma, np.float64)
+        dL = np.ascontiguousarray(dLambda, np.float64)
+        if dG.size != dL.size or not 1 <= dG.size <= 2:
+            raise ValueError("Batch.tangent_init: one or two directions, dGamma and dLambda of equal length")
+        self._tanK = int(dG.size)
+        _check(lib().dmfe_batch_tangent_init(self._h, C.c_int(self._tanK), _ptr(dG), _ptr(dL)),
+               "dmfe_batch_tangent_init")
+
+    def tangent_observables(self):
+        """(dE, dQK0), each (n, K): derivatives of E(t) and QK(t,0) at the proposed t of
+        every instance's last step along the directions of tangent_init."""
+        K = getattr(self, "_tanK", 0)
+        if K < 1:
+            raise DmfeError("Batch.tangent_observables: call tangent_init first")
+        dE, dQ = np.zeros(self.n * K), np.zeros(self.n * K)
+        _check(lib().dmfe_batch_tangent_observables(self._h, _ptr(dE), _ptr(dQ)), "dmfe_batch_tangent_observables")
+        return dE.reshape(self.n, K), dQ.reshape(self.n, K)
 
     def step(self):
         """One batch step (direct launches); returns the number of running instances."""

@@ -75,7 +75,9 @@ struct Inst {
     Work W{};
     int split = 1;
     int64_t cap = 0;
-    TanArgs X{};  // tangents (device loop of a context only; X.K = 0 otherwise)
+    TanArgs X{};  // tangents (dmfe_batch_tangent_init, or a context's device loop); X.K = 0: none
+    History tH[kMaxTan] = {};  // the tangent histories this batch owns (dmfe_batch_tangent_init)
+    double* tWork = nullptr;   // their work arrays
 };
 
 }  // namespace
@@ -201,11 +203,17 @@ int enqueue_step(dmfe_batch* b, int S) {
         return DMFE_OK;
     }
     launch_batch_begin(b->d_desc, n, b->stream);
+    const int K = b->tanK;  // tangent directions carried by every instance (0: none)
     for (int k = 0; k < S; ++k) {
-        if (k > 0 || s0) launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);
-        launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->maxm, b->lds_rhs, b->stream);
+        if (k > 0 || s0) {
+            if (K) launch_batch_stage_tan(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, K, b->stream);
+            else launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, k, b->ept, b->stream);
+        }
+        if (K) launch_batch_rhs_tan(b->d_desc, b->d_rmap, b->nb_rhs, k, b->G, K, b->lds_rhs, b->stream);
+        else launch_batch_rhs(b->d_desc, b->d_rmap, b->nb_rhs, k, b->log_response, b->G, b->maxm, b->lds_rhs, b->stream);
     }
-    launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, S, b->ept, b->stream);
+    if (K) launch_batch_stage_tan(b->d_desc, b->d_smap, b->nb_stage, S, b->ept, K, b->stream);
+    else launch_batch_stage(b->d_desc, b->d_smap, b->nb_stage, S, b->ept, b->stream);
     launch_batch_end(b->d_desc, n, b->stream);
     if (b->obs_on) launch_batch_obs(b->d_desc, n, b->d_obs, b->obs_cap, b->stream);
     HIP_TRY(hipGetLastError());
@@ -350,7 +358,9 @@ int service(dmfe_batch* b, bool* changed) {
         Ctl& c = b->ctl[(size_t)i];
         int rc = DMFE_OK;
         if (b->inst[(size_t)i].X.K && (c.status == kNeedMaint || c.status == kNeedRows))
-            return set_error(DMFE_ERR_STATE, "device loop with tangents: arena or maintenance point reached");
+            return set_error(DMFE_ERR_STATE, "instance " + std::to_string(i) +
+                                                 " with tangents: history arena full or maintenance point reached "
+                                                 "(size the arena for the run; tangents do not cross a sparsification)");
         if (c.status == kNeedMaint) rc = maintenance(b, i);
         else if (c.status == kNeedRows) rc = grow_inst(b, i);
         else continue;
@@ -382,6 +392,8 @@ void free_batch(dmfe_batch* b) {
         for (int f = 0; f < 3; ++f) { (void)hipFree(g.inds[f]); (void)hipFree(g.w[f]); }
     }
     for (Inst& x : b->inst) {
+        for (History& h : x.tH) free_hist(h);
+        (void)hipFree(x.tWork);
         free_hist(x.H);
         Work& W = x.W;
         for (void* p : {(void*)W.posB1, (void*)W.posB2, (void*)W.saveB1, (void*)W.saveB2, (void*)W.Floor, (void*)W.diff,
@@ -731,6 +743,17 @@ extern "C" int dmfe_batch_size(const dmfe_batch* b, int* n) {
 extern "C" int dmfe_batch_init_quench(dmfe_batch* b) {
     if (!b) return set_error(DMFE_ERR_ARG, "null batch");
     drop_graphs(b);
+    if (b->tanK) {  // sensitivities start again with dmfe_batch_tangent_init
+        for (Inst& x : b->inst) {
+            for (History& h : x.tH) free_hist(h);
+            (void)hipFree(x.tWork);
+            x.tWork = nullptr;
+            x.X = TanArgs{};
+        }
+        b->tanK = 0;
+        int rc = upload_desc(b);
+        if (rc) return rc;
+    }
     for (int i = 0; i < n_inst(b); ++i) {
         Inst& x = b->inst[(size_t)i];
         Ctl& c = b->ctl[(size_t)i];
@@ -764,6 +787,97 @@ extern "C" int dmfe_batch_init_quench(dmfe_batch* b) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->steps_run = 0;
     return push_ctl(b);
+}
+
+// Forward-mode sensitivities for every instance (the batch form of dmfe_tangent_init):
+// K directions (dGamma[k], dLambda[k]), the same for all instances, carried through the
+// batch's stage and RHS launches.  Right after dmfe_batch_init_quench; the tangent
+// histories take the batch's arena size, which must hold the whole run.
+extern "C" int dmfe_batch_tangent_init(dmfe_batch* b, int K, const double* dGamma, const double* dLambda) {
+    if (!b || K < 1 || K > kMaxTan || !dGamma || !dLambda)
+        return set_error(DMFE_ERR_ARG, "dmfe_batch_tangent_init: bad argument (1 <= K <= 2)");
+    if (b->log_response)
+        return set_error(DMFE_ERR_STATE, "dmfe_batch_tangent_init: the log-response interpolation is not differentiated");
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    for (const Ctl& c : b->ctl)
+        if (c.T1 != 1 || c.status != kRunning)
+            return set_error(DMFE_ERR_STATE, "dmfe_batch_tangent_init: call right after dmfe_batch_init_quench");
+    drop_graphs(b);
+    b->lds_rhs = 0;
+    for (int i = 0; i < n_inst(b); ++i) {
+        Inst& x = b->inst[(size_t)i];
+        const int N = b->grids[(size_t)x.gi].N;
+        for (History& h : x.tH) free_hist(h);
+        (void)hipFree(x.tWork);
+        x.tWork = nullptr;
+        x.X = TanArgs{};
+        for (int k = 0; k < K; ++k) {
+            rc = alloc_hist(x.tH[k], x.cap, N);
+            if (rc) return rc;
+        }
+        // partial [K][N][split][8] | qkb1c0 [K][N] | hK, hR [K][10][N] | scal [K][8]
+        const size_t nd = (size_t)K * ((size_t)N * x.split * kSums + N + 2 * (size_t)kMaxStages * N + 8);
+        if (hipMalloc(&x.tWork, nd * sizeof(double)) != hipSuccess)
+            return set_error(DMFE_ERR_NOMEM, "dmfe_batch_tangent_init: work allocation failed");
+        HIP_TRY(hipMemsetAsync(x.tWork, 0, nd * sizeof(double), b->stream));
+        TanArgs& X = x.X;
+        X.K = K;
+        for (int k = 0; k < K; ++k) {
+            X.dG[k] = dGamma[k];
+            X.dL[k] = dLambda[k];
+            X.V[k] = x.tH[k].V;
+            X.D[k] = x.tH[k].D;
+            X.r[k] = x.tH[k].r;
+            X.dr[k] = x.tH[k].dr;
+        }
+        double* p = x.tWork;
+        X.partial = p; p += (size_t)K * N * x.split * kSums;
+        X.qkb1c0 = p; p += (size_t)K * N;
+        X.hK = p; p += (size_t)K * kMaxStages * N;
+        X.hR = p; p += (size_t)K * kMaxStages * N;
+        X.scal = p;
+        launch_tan_init_n(x.M, X, N, b->stream);
+        HIP_TRY(hipGetLastError());
+        b->lds_rhs = std::max(b->lds_rhs, rhs_lds_bytes(N, (N + x.split - 1) / x.split, K));
+    }
+    b->tanK = K;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return upload_desc(b);
+}
+
+// d(E(t), QK(t,0)) along each of the K directions for every instance, at the proposed t
+// of its last step (the time of its correlation line, simulation_runner.cpp:327-341, and
+// of dmfe_gradients; t1grid.back() before the first step): dE[i * K + k], dQK0[i * K + k]
+// (the batch form of dmfe_tangent_observables).
+extern "C" int dmfe_batch_tangent_observables(dmfe_batch* b, double* dE, double* dQK0) {
+    if (!b) return set_error(DMFE_ERR_ARG, "dmfe_batch_tangent_observables: null batch");
+    if (b->tanK < 1) return set_error(DMFE_ERR_STATE, "dmfe_batch_tangent_observables: no sensitivities carried");
+    int rc = pull_ctl(b);
+    if (rc) return rc;
+    const int K = b->tanK;
+    for (int i = 0; i < n_inst(b); ++i) {
+        const Inst& x = b->inst[(size_t)i];
+        const GridDev& gd = b->grids[(size_t)x.gi];
+        const int T1 = b->ctl[(size_t)i].T1;
+        double t = b->ctl[(size_t)i].t_prop;  // the proposed t of the last step, as dmfe_gradients
+        if (b->ctl[(size_t)i].loop == 0) {     // no step yet: t1grid.back()
+            HIP_TRY(hipMemcpyAsync(&t, x.H.t1 + (T1 - 1), sizeof(double), hipMemcpyDeviceToHost, b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+        }
+        launch_tan_energy(gd.tables(), x.H, x.M, x.X, T1 - 1, t, b->stream);
+        HIP_TRY(hipGetLastError());
+        for (int k = 0; k < K; ++k) {
+            double e = 0.0, q = 0.0;
+            HIP_TRY(hipMemcpyAsync(&e, x.X.scal + k * 8 + 2, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+            HIP_TRY(hipMemcpyAsync(&q, &x.X.V[k][(size_t)(T1 - 1) * gd.N].x, sizeof(double), hipMemcpyDeviceToHost,
+                                   b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            if (dE) dE[(size_t)i * K + k] = e;
+            if (dQK0) dQK0[(size_t)i * K + k] = q;
+        }
+    }
+    return DMFE_OK;
 }
 
 // One batch step for every running instance (direct launches, then a synchronisation).

@@ -2047,6 +2047,49 @@ __global__ __launch_bounds__(kStageThreads) void loop_stage_tan_kernel(InstDesc 
     stage_role_body<EPT, true>(d, role, n, lds, red);
 }
 
+// Stage launch n of a batch whose instances carry K tangent directions: the finish role
+// runs the tangent finish of stage n-1 before the primal finish (the order of
+// loop_stage_tan_kernel and of the host loop), the search roles are those of
+// batch_stage_kernel.
+template <int EPT, int K>
+__global__ __launch_bounds__(kStageThreads) void batch_stage_tan_kernel(const InstDesc* __restrict__ D,
+                                                                        const int2* __restrict__ map, int n) {
+    __shared__ double lds[EPT * kStageThreads];
+    __shared__ double red[(6 * K > 8 ? 6 * K : 8) * (kStageThreads / kWave)];
+    const int2 mp = map[blockIdx.x];
+    if (mp.x < 0) return;
+    const InstDesc& d = D[mp.x];
+    if (mp.y == -2) {
+        if (n < 1) return;
+        const Ctl& c = *d.ctl;
+        const int active = c.active, S = c.S;
+        if (!active || n > S) return;
+        tan_finish_body<EPT, K>(d.T, d.H, d.W, d.M, c.fin[n - 1], d.X, red);
+        __syncthreads();
+        finish_block<EPT, false>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
+        return;
+    }
+    stage_role_body<EPT, false>(d, mp.y, n, lds, red);
+}
+
+// RHS launch n of a batch whose instances carry K tangent directions (linear-response
+// form): the primal quadratures and the tangent quadratures of every direction in one pass,
+// block map as in batch_rhs_kernel.
+template <int G, int K>
+__global__ __launch_bounds__(kRowThreads) void batch_rhs_tan_kernel(const InstDesc* __restrict__ D,
+                                                                    const int4* __restrict__ map, int n) {
+    extern __shared__ double2 dyn[];
+    __shared__ double red[kSums * (kRowThreads / kWave)];
+    const int4 mp = map[xcd_stripe<16>(blockIdx.x, gridDim.x)];
+    if (mp.x < 0) return;
+    const InstDesc& d = D[mp.x];
+    const Ctl& c = *d.ctl;
+    const int active = c.active, S = c.S;
+    const RhsArgs a = step_rhs_args(c, d, n, 0);  // loaded with the control word
+    if (!active || n >= S) return;
+    rhs_rows_body<false, G, K>(d.T, d.H, d.W, d.M, a, mp.y, mp.z, dyn, red, &d.X);
+}
+
 // Tangent of the energy at time t on history row `row` (simulation_runner.cpp:308-318)
 template <int K>
 __global__ __launch_bounds__(kStageThreads) void tan_energy_kernel(Tables T, History H, Model M, TanArgs X, int row,

@@ -294,6 +294,9 @@ void launch_batch_stage(InstDesc* D, const int2* map, int nblocks, int n, int ep
 void launch_batch_rhs(InstDesc* D, const int4* map, int nblocks, int n, int log_response, int G, int maxm, size_t lds,
                       hipStream_t s);
 void launch_batch_obs(InstDesc* D, int n, double* obs, int obs_cap, hipStream_t s);
+// the stage and RHS launches of a batch whose instances carry K tangent directions
+void launch_batch_stage_tan(InstDesc* D, const int2* map, int nblocks, int n, int ept, int K, hipStream_t s);
+void launch_batch_rhs_tan(InstDesc* D, const int4* map, int nblocks, int n, int G, int K, size_t lds, hipStream_t s);
 // the step launches of one context's device loop, the instance passed by value
 void launch_loop_begin(const InstDesc& d, hipStream_t s);
 void launch_loop_end(const InstDesc& d, hipStream_t s);

@@ -35,4 +35,34 @@ void launch_loop_rhs(const InstDesc& d, int n, int log_response, int G, size_t l
 }
 
 
+// Batch launches with K tangent directions per instance (batch_stage_tan_kernel,
+// batch_rhs_tan_kernel).
+void launch_batch_stage_tan(InstDesc* D, const int2* map, int nblocks, int n, int ept, int K, hipStream_t s) {
+#define DMFE_BST(E, KK) hipLaunchKernelGGL((batch_stage_tan_kernel<E, KK>), dim3(nblocks), dim3(kStageThreads), 0, s, D, map, n)
+#define DMFE_BSTE(KK)                       \
+    if (ept <= 1) DMFE_BST(1, KK);          \
+    else if (ept <= 2) DMFE_BST(2, KK);     \
+    else if (ept <= 4) DMFE_BST(4, KK);     \
+    else DMFE_BST(8, KK);
+    if (K == 1) { DMFE_BSTE(1) }
+    else { DMFE_BSTE(2) }
+#undef DMFE_BSTE
+#undef DMFE_BST
+}
+
+void launch_batch_rhs_tan(InstDesc* D, const int4* map, int nblocks, int n, int G, int K, size_t lds, hipStream_t s) {
+#define DMFE_BRT(GG, KK) hipLaunchKernelGGL((batch_rhs_tan_kernel<GG, KK>), dim3(nblocks), dim3(kRowThreads), lds, s, D, map, n)
+#define DMFE_BRTG(KK)                       \
+    switch (G) {                            \
+        case 4: DMFE_BRT(4, KK); break;     \
+        case 8: DMFE_BRT(8, KK); break;     \
+        case 16: DMFE_BRT(16, KK); break;   \
+        default: DMFE_BRT(64, KK); break;   \
+    }
+    if (K == 1) DMFE_BRTG(1)
+    else DMFE_BRTG(2)
+#undef DMFE_BRTG
+#undef DMFE_BRT
+}
+
 }  // namespace dmfe

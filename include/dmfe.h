@@ -369,6 +369,19 @@ int dmfe_batch_set_limits(dmfe_batch* b, double tmax, int64_t max_loop);
 /* One batch step with direct launches; *running = instances still running. */
 int dmfe_batch_step(dmfe_batch* b, int* running);
 
+/* Forward-mode sensitivities of every instance, carried inside the batch's launches:
+ * K (1..2) directions (dGamma[k], dLambda[k]), the same for all instances (the batch form
+ * of dmfe_tangent_init; no reference counterpart, see dmfe_gradients).  Call right after
+ * dmfe_batch_init_quench (which also drops them); linear-response form only.  The tangent
+ * histories take the batch's arena size: a run that would grow the arena or reach the
+ * every-1e5-loops maintenance stops with DMFE_ERR_STATE. */
+int dmfe_batch_tangent_init(dmfe_batch* b, int K, const double* dGamma, const double* dLambda);
+
+/* d(E(t), QK(t,0)) along the K directions for every instance at the proposed t of its
+ * last step (as dmfe_gradients; t1grid.back() before any step): dE[i * K + k],
+ * dQK0[i * K + k] (the batch form of dmfe_tangent_observables). */
+int dmfe_batch_tangent_observables(dmfe_batch* b, double* dE, double* dQK0);
+
 /* Up to max_steps batch steps, stopping early when no instance runs.  use_graph:
  * replay a hipGraph of `chunk` captured steps (host checks between replays);
  * otherwise launch directly.  *steps_out = batch steps run. */
