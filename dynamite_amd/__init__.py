@@ -131,8 +131,10 @@ def stencil_width(N, method=INTERP_POLY, order=9, fh_stencil=0):
 
 
 def generate_grid(N, Tmax=1e5, spline_order=5, method=INTERP_POLY, order=9, fh_stencil=0, alpha=0.0, delta=0.0,
-                  with_pos=False):
-    """Grid package generator (the reference's `grid` subcommand, grid_cli.cpp:35-233)."""
+                  with_pos=False, device=False):
+    """Grid package generator (the reference's `grid` subcommand, grid_cli.cpp:35-233).
+    device=True forms the dense B-spline weights on the current HIP device
+    (dmfe_grid_generate_device)."""
     spec = GridSpec(N, Tmax, alpha, delta, spline_order, method, order, fh_stencil)
     m = stencil_width(N, method, order, fh_stencil)
     P = N * N
@@ -141,9 +143,10 @@ def generate_grid(N, Tmax=1e5, spline_order=5, method=INTERP_POLY, order=9, fh_s
     pos = (np.zeros(P), np.zeros(P), np.zeros(P)) if with_pos else (None, None, None)
     inds = np.zeros(3 * P, np.int64)
     weights = np.zeros(3 * P * m)
-    rc = lib().dmfe_grid_generate(C.byref(spec), _ptr(theta), _ptr(integ), _ptr(phi1), _ptr(phi2), _ptr(pos[0]),
-                                  _ptr(pos[1]), _ptr(pos[2]), _ptr(inds), _ptr(weights))
-    _check(rc, "dmfe_grid_generate")
+    gen = lib().dmfe_grid_generate_device if device else lib().dmfe_grid_generate
+    rc = gen(C.byref(spec), _ptr(theta), _ptr(integ), _ptr(phi1), _ptr(phi2), _ptr(pos[0]), _ptr(pos[1]),
+             _ptr(pos[2]), _ptr(inds), _ptr(weights))
+    _check(rc, "dmfe_grid_generate_device" if device else "dmfe_grid_generate")
     return GridPackage(N, m, theta, integ, phi2, inds, weights, phi1=phi1, pos=pos if with_pos else None, spec=spec)
 
 

@@ -349,21 +349,16 @@ void grid_rational_stencils(const std::vector<long double>& x, const std::vector
     }
 }
 
-// Dense interpolating B-spline of degree p through all N nodes: the weights of
-// query q solve A^T w = e(q), with A_ij = B_j(x_i) and e(q) the p+1 non-zero
-// basis values at q.  A^T is LU-factorised once (partial pivoting, long double)
-// and every query is a forward/back substitution, as in
-// interpolation_weights.cpp:401-456; queries are spread over host threads
-// (each query's arithmetic is unchanged, so results do not depend on the split).
-void grid_bspline_weights(const std::vector<long double>& x, const std::vector<long double>& q, int p,
-                          double* W) {
+// LU factorisation with partial pivoting (long double) of A^T, A_ij = B_j(x_i) the
+// collocation matrix at the averaged knots t: the system every dense B-spline weight solves.
+void bspline_lu(const std::vector<long double>& x, int p, const std::vector<ld>& t, std::vector<ld>& LU,
+                std::vector<int>& piv) {
     const int N = (int)x.size();
-    const std::vector<ld> t = averaged_knots(x, p);
     const std::vector<ld> A = collocation(x, p, t);
-    std::vector<ld> LU((std::size_t)N * N);
+    LU.assign((std::size_t)N * N, 0.0L);
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < N; ++j) LU[(std::size_t)i * N + j] = A[(std::size_t)j * N + i];
-    std::vector<int> piv(N);
+    piv.resize(N);
     for (int i = 0; i < N; ++i) piv[i] = i;
     for (int k = 0; k < N; ++k) {
         int r = k;
@@ -383,6 +378,21 @@ void grid_bspline_weights(const std::vector<long double>& x, const std::vector<l
             for (int j = k + 1; j < N; ++j) LU[(std::size_t)i * N + j] -= l * LU[(std::size_t)k * N + j];
         }
     }
+}
+
+// Dense interpolating B-spline of degree p through all N nodes: the weights of
+// query q solve A^T w = e(q), with A_ij = B_j(x_i) and e(q) the p+1 non-zero
+// basis values at q.  A^T is LU-factorised once (partial pivoting, long double)
+// and every query is a forward/back substitution, as in
+// interpolation_weights.cpp:401-456; queries are spread over host threads
+// (each query's arithmetic is unchanged, so results do not depend on the split).
+void grid_bspline_weights(const std::vector<long double>& x, const std::vector<long double>& q, int p,
+                          double* W) {
+    const int N = (int)x.size();
+    const std::vector<ld> t = averaged_knots(x, p);
+    std::vector<ld> LU;
+    std::vector<int> piv;
+    bspline_lu(x, p, t, LU, piv);
     const std::size_t Q = q.size();
     auto work = [&](std::size_t q0, std::size_t q1) {
         std::vector<ld> rhs(N), sol(N), B;
@@ -410,6 +420,89 @@ void grid_bspline_weights(const std::vector<long double>& x, const std::vector<l
     std::vector<std::thread> th;
     for (unsigned i = 0; i < nt; ++i) th.emplace_back(work, Q * i / nt, Q * (i + 1) / nt);
     for (auto& x_ : th) x_.join();
+}
+
+// The same weights formed on the current HIP device.  w(q) = (A^T)^{-1} e(q) is the
+// combination of the p+1 rows of Z = ((A^T)^{-1})^T picked by the non-zero basis values of
+// q: Z is formed once on the host in long double (N solves of the LU factorisation with
+// unit vectors, over host threads) and rounded to double; the host evaluates each query's
+// knot span and basis values; the device forms the N^2 combinations of N entries per table
+// (launch_bspline_combine) in chunks and the rows are copied into W.  The explicit inverse
+// and the double-precision combination change the last bits against the per-query solve
+// (tested against it and by polynomial reproduction).
+void grid_bspline_weights_device(const std::vector<long double>& x, const std::vector<long double>& q, int p,
+                                 double* W) {
+    const int N = (int)x.size();
+    const std::vector<ld> t = averaged_knots(x, p);
+    std::vector<ld> LU;
+    std::vector<int> piv;
+    bspline_lu(x, p, t, LU, piv);
+    std::vector<double> Z((std::size_t)N * N);  // row c: column c of (A^T)^{-1}
+    unsigned nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    auto inv = [&](int c0, int c1) {
+        std::vector<ld> sol(N);
+        for (int c = c0; c < c1; ++c) {
+            for (int i = 0; i < N; ++i) sol[i] = (piv[i] == c) ? 1.0L : 0.0L;
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < i; ++j) sol[i] -= LU[(std::size_t)i * N + j] * sol[j];
+            for (int i = N - 1; i >= 0; --i) {
+                for (int j = i + 1; j < N; ++j) sol[i] -= LU[(std::size_t)i * N + j] * sol[j];
+                sol[i] /= LU[(std::size_t)i * N + i];
+            }
+            for (int i = 0; i < N; ++i) Z[(std::size_t)c * N + i] = (double)sol[i];
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned i = 0; i < nt; ++i) th.emplace_back(inv, (int)((long long)N * i / nt), (int)((long long)N * (i + 1) / nt));
+        for (auto& x_ : th) x_.join();
+    }
+    const std::size_t Q = q.size();
+    const int nb = p + 1;
+    std::vector<int> start(Q);
+    std::vector<double> basis(Q * (std::size_t)nb);
+    auto spans = [&](std::size_t q0, std::size_t q1) {
+        std::vector<ld> B;
+        for (std::size_t k = q0; k < q1; ++k) {
+            const int sp = knot_span(N, p, q[k], t);
+            basis_values(sp, q[k], p, t, B);
+            int s0 = sp - p;
+            for (int j = 0; j <= p; ++j) {  // columns outside [0, N) carry no weight
+                const int col = s0 + j;
+                basis[k * nb + j] = (col >= 0 && col < N) ? (double)B[j] : 0.0;
+            }
+            start[k] = s0;
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned i = 0; i < nt; ++i) th.emplace_back(spans, Q * i / nt, Q * (i + 1) / nt);
+        for (auto& x_ : th) x_.join();
+    }
+    double *dZ = nullptr, *dB = nullptr, *dW = nullptr;
+    int* dS = nullptr;
+    const std::size_t chunk = std::min<std::size_t>(Q, std::max<std::size_t>(1, (std::size_t)(256u << 20) / ((std::size_t)N * 8)));
+    auto check = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess) {
+            (void)hipFree(dZ); (void)hipFree(dB); (void)hipFree(dW); (void)hipFree(dS);
+            throw std::runtime_error(std::string("B-spline weights on the device: ") + what + ": " + hipGetErrorString(e));
+        }
+    };
+    check(hipMalloc(&dZ, Z.size() * sizeof(double)), "hipMalloc");
+    check(hipMalloc(&dB, basis.size() * sizeof(double)), "hipMalloc");
+    check(hipMalloc(&dS, start.size() * sizeof(int)), "hipMalloc");
+    check(hipMalloc(&dW, chunk * N * sizeof(double)), "hipMalloc");
+    check(hipMemcpy(dZ, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+    check(hipMemcpy(dB, basis.data(), basis.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+    check(hipMemcpy(dS, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice), "upload");
+    for (std::size_t q0 = 0; q0 < Q; q0 += chunk) {
+        const std::size_t nq = std::min(chunk, Q - q0);
+        launch_bspline_combine(dZ, dS + q0, dB + q0 * nb, p, N, (long long)nq, dW, nullptr);
+        check(hipGetLastError(), "launch");
+        check(hipMemcpy(W + q0 * N, dW, nq * N * sizeof(double), hipMemcpyDeviceToHost), "download");
+    }
+    check(hipDeviceSynchronize(), "synchronize");
+    (void)hipFree(dZ); (void)hipFree(dB); (void)hipFree(dW); (void)hipFree(dS);
 }
 
 // ---- pos grids (theta value -> fractional 1-based index), written for format parity ----
@@ -529,9 +622,9 @@ extern "C" int dmfe_grid_theta_ld(int64_t N, double Tmax, double alpha, double d
     }
 }
 
-extern "C" int dmfe_grid_generate(const dmfe_grid_spec* spec, double* theta, double* integ, double* phi1,
-                                  double* phi2, double* posA1, double* posA2, double* posB2,
-                                  int64_t* inds, double* weights) {
+namespace {
+int grid_generate(const dmfe_grid_spec* spec, double* theta, double* integ, double* phi1, double* phi2, double* posA1,
+                  double* posA2, double* posB2, int64_t* inds, double* weights, bool device) {
     try {
         if (!spec || spec->N < 2 || !theta || !integ || !phi2 || !inds || !weights)
             return set_error(DMFE_ERR_ARG, "dmfe_grid_generate: null or invalid argument");
@@ -573,13 +666,30 @@ extern "C" int dmfe_grid_generate(const dmfe_grid_spec* spec, double* theta, dou
                 dmfe::grid_rational_stencils(th, *qs[f], n, (int)m, st, wt);
             } else {
                 for (std::size_t k = 0; k < P; ++k) st[k] = -1;
-                dmfe::grid_bspline_weights(th, *qs[f], n, wt);
+                if (device) dmfe::grid_bspline_weights_device(th, *qs[f], n, wt);
+                else dmfe::grid_bspline_weights(th, *qs[f], n, wt);
             }
         }
         return DMFE_OK;
     } catch (const std::exception& e) {
         return set_error(DMFE_ERR_ARG, std::string("dmfe_grid_generate: ") + e.what());
     }
+}
+}  // namespace
+
+extern "C" int dmfe_grid_generate(const dmfe_grid_spec* spec, double* theta, double* integ, double* phi1,
+                                  double* phi2, double* posA1, double* posA2, double* posB2,
+                                  int64_t* inds, double* weights) {
+    return grid_generate(spec, theta, integ, phi1, phi2, posA1, posA2, posB2, inds, weights, false);
+}
+
+extern "C" int dmfe_grid_generate_device(const dmfe_grid_spec* spec, double* theta, double* integ, double* phi1,
+                                         double* phi2, double* posA1, double* posA2, double* posB2,
+                                         int64_t* inds, double* weights) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        return set_error(DMFE_ERR_HIP, "dmfe_grid_generate_device: no HIP device");
+    return grid_generate(spec, theta, integ, phi1, phi2, posA1, posA2, posB2, inds, weights, true);
 }
 
 // ---- text formats (grid_io.cpp:26-328, io_input.cpp:60-110) -------------------------

@@ -22,6 +22,9 @@ void grid_poly_stencils(const std::vector<long double>& x, const std::vector<lon
 void grid_rational_stencils(const std::vector<long double>& x, const std::vector<long double>& q, int d, int m,
                             int64_t* starts, double* alpha);
 void grid_bspline_weights(const std::vector<long double>& x, const std::vector<long double>& q, int p, double* W);
+// the same weights formed on the current HIP device (launch_bspline_combine)
+void grid_bspline_weights_device(const std::vector<long double>& x, const std::vector<long double>& q, int p,
+                                 double* W);
 
 constexpr int kMaxN = 2048;     // LDS sizing of the row kernels
 constexpr int kMaxStages = 10;  // SSPRK(10,4)
@@ -197,6 +200,9 @@ void launch_gather_rows(const Tables& T, const History& src, const History& dst,
                         hipStream_t s);
 
 void launch_hbm_copy(const double2* src, double2* dst, long long n, hipStream_t s);
+// W[k][i] = sum_{j <= p} B[k][j] Z[start[k] + j][i], k < Q, i < N (dense B-spline weights)
+void launch_bspline_combine(const double* Z, const int* start, const double* B, int p, int N, long long Q, double* W,
+                            hipStream_t s);
 
 // Device scratch of the no-allocation RHS entry point (dmfe_rhs_noalloc): kScratchDoubles
 // doubles (1 MiB) in a file-scope device array, one per device.  scratch_address() gives

@@ -156,6 +156,31 @@ int scratch_address(double** out) {
     return DMFE_OK;
 }
 
+// Dense B-spline weights of the grid generator (grid_bspline_weights_device):
+// W[k][i] = sum_{j <= p} B[k][j] Z[start[k] + j][i] for queries k < Q, i < N; one block per
+// query, rows of Z read coalesced, columns outside [0, N) skipped (their basis value is 0).
+__global__ __launch_bounds__(256) void bspline_combine_kernel(const double* __restrict__ Z, const int* __restrict__ start,
+                                                              const double* __restrict__ B, int p, int N,
+                                                              double* __restrict__ W) {
+    const long long k = blockIdx.x;
+    const int s0 = start[k];
+    const double* b = B + k * (p + 1);
+    for (int i = threadIdx.x; i < N; i += 256) {
+        double acc = 0.0;
+        for (int j = 0; j <= p; ++j) {
+            const int c = s0 + j;
+            if (c >= 0 && c < N) acc = fma(b[j], Z[(size_t)c * N + i], acc);
+        }
+        W[(size_t)k * N + i] = acc;
+    }
+}
+
+void launch_bspline_combine(const double* Z, const int* start, const double* B, int p, int N, long long Q, double* W,
+                            hipStream_t s) {
+    if (Q <= 0) return;
+    hipLaunchKernelGGL(bspline_combine_kernel, dim3((unsigned)Q), dim3(256), 0, s, Z, start, B, p, N, W);
+}
+
 void launch_dense_apply(const double* W, const double* X, double* Y, long long P, int K, int B, int ldy,
                         hipStream_t s) {
     const long long nb = (P + kDenseRows - 1) / kDenseRows;

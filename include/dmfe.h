@@ -78,6 +78,15 @@ int64_t dmfe_grid_stencil_width(int64_t N, int method, int order, int fh_stencil
 int dmfe_grid_generate(const dmfe_grid_spec* spec, double* theta, double* integ, double* phi1, double* phi2,
                        double* posA1, double* posA2, double* posB2, int64_t* inds, double* weights);
 
+/* dmfe_grid_generate with the dense B-spline weights (method bspline, the N^4 part of the
+ * generator, interpolation_weights.cpp:401-456) formed on the current HIP device: the
+ * inverse collocation matrix once on the host (long double), the N^2 queries of each
+ * table as combinations of its rows on the device.  Other methods and all other tables
+ * are identical to dmfe_grid_generate; the dense weights differ from its per-query solve
+ * in the last bits.  DMFE_ERR_HIP without a device. */
+int dmfe_grid_generate_device(const dmfe_grid_spec* spec, double* theta, double* integ, double* phi1, double* phi2,
+                              double* posA1, double* posA2, double* posB2, int64_t* inds, double* weights);
+
 /* theta grid in extended precision (long double, the generator's internal
  * precision; theta_grid.cpp:71-125). */
 int dmfe_grid_theta_ld(int64_t N, double Tmax, double alpha, double delta, long double* theta);

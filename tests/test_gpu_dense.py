@@ -136,3 +136,34 @@ def test_dense_contraction_full_size_N512():
         print(f"dense N=512 B=4 map {map_}: {1e3 * ms:.1f} us, max |Y-ref| / bound = {err:.3f}", flush=True)
         assert np.all(np.abs(Y - ref) <= bound + 1e-300), err
     ctx.close()
+
+
+@pytest.mark.parametrize("N,Tmax,order", [(64, 1e2, 5), (96, 1e2, 5), (128, 1e2, 7)])
+def test_device_bspline_weights_match_host_solve(N, Tmax, order):
+    """dmfe_grid_generate_device: the dense B-spline weights formed on the device (rows of
+    the long-double inverse collocation matrix combined per query) against the host's
+    per-query LU solve (interpolation_weights.cpp:401-456): every other table identical,
+    the weights equal to rounding on the scale of the Lebesgue constant (max_q sum |w_q|),
+    and both reproduce polynomials of the spline degree at the query points to that scale.
+    The grids are well conditioned; on the configuration's family (Tmax 1e5, degree 9) the
+    collocation matrix is numerically singular (Lebesgue constants ~1e36 at N = 128), in
+    the reference's algorithm and here alike, and the weights of both solves are rounding
+    noise there."""
+    host = dm.generate_grid(N, Tmax, 5, dm.INTERP_BSPLINE, order, 0)
+    dev = dm.generate_grid(N, Tmax, 5, dm.INTERP_BSPLINE, order, 0, device=True)
+    assert np.array_equal(host.theta, dev.theta) and np.array_equal(host.integ, dev.integ)
+    assert np.array_equal(host.phi2, dev.phi2) and np.array_equal(host.inds, dev.inds)
+    P = N * N
+    th = np.asarray(host.theta)
+    qs = [np.asarray(host.phi1).reshape(P), np.asarray(host.phi2).reshape(P)]
+    for f in range(3):
+        wh = np.asarray(host.weights[f]).reshape(P, N)
+        wd = np.asarray(dev.weights[f]).reshape(P, N)
+        leb = np.abs(wh).sum(axis=1).max()
+        err = np.abs(wd - wh).max()
+        print(f"N={N} Tmax={Tmax:g} degree {order} table {f}: Lebesgue {leb:.3g}, max |dW| {err:.2e}", flush=True)
+        assert err <= 1e-12 * leb, (f, err, leb)
+        if f < 2:  # A1, A2 query points phi1, phi2: reproduction of x^k, k <= degree
+            for k in range(order + 1):
+                got = wd @ th ** k
+                assert np.abs(got - qs[f] ** k).max() <= 1e-12 * leb, (f, k)
