@@ -43,9 +43,21 @@ struct or_ctx {
     int rk_init;
     int use_serk2; /* the GPU loop's SERK2 promotion at the maintenance (config.use_serk2) */
     long maint_every; /* the maintenance cadence, 100000 in the reference (0: 100000) */
+    int rollback;     /* the GPU loop's rollback (simulation_runner.cpp:280-303), off = CPU path */
+    long last_rollback; /* last_rollback_loop (simulation_runner.cpp:53), -1000 until a rollback */
+    /* QK tail fit near theta -> 1 (tail_fit.cpp:46-265, --tail-fit): on/off and the state the
+     * reference keeps in function-local statics (prev_best_off, prev_t1len, alpha_smoothed,
+     * last_known_rollback), here per context */
+    int tail_fit;
+    long tf_prev_off; /* -1: SIZE_MAX sentinel, no window chosen yet */
+    size_t tf_prev_t1len;
+    double tf_alpha;
+    long tf_known_rollback;
+    long tf_applied; /* blends applied (not in the reference: a test observable) */
 };
 
 static double* dalloc(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
+static void tail_fit_blend(or_ctx* c);
 
 /* ---- math_ops.cpp:7-37 ------------------------------------------------------ */
 static double pow_int(double base, int e) {
@@ -810,6 +822,7 @@ double or_update(or_ctx* c) {
 
 void or_set_serk2(or_ctx* c, int on) { c->use_serk2 = on; }
 void or_set_maint_every(or_ctx* c, long every) { c->maint_every = every; }
+void or_set_rollback(or_ctx* c, int on) { c->rollback = on; }
 
 /* ---- sparsify_utils.cpp:14-97 (CPU-only build) ------------------------------------ */
 void or_sparsify(or_ctx* c, double threshold) {
@@ -870,9 +883,11 @@ double or_energy(or_ctx* c, double t) {
 /* ---- simulation_runner.cpp:125-278 (CPU-only branches, no I/O) ------------------
  * Returns the proposed time t written to correlation.txt.                           */
 double or_loop_iteration(or_ctx* c) {
-    const double t = c->t1[c->T1 - 1] + c->delta_t_val;
+    double t = c->t1[c->T1 - 1] + c->delta_t_val;
     c->delta_old = c->delta;
     c->delta = or_update(c);
+    /* the tail fit runs after update() and before loop++ (simulation_runner.cpp:135-141) */
+    if (c->tail_fit) tail_fit_blend(c);
     c->loop++;
     if (c->loop % (c->maint_every > 0 ? c->maint_every : 100000) == 0) {
         size_t prev = c->T1;
@@ -885,7 +900,7 @@ double or_loop_iteration(or_ctx* c) {
         }
         interpolate_with(c, 0, NULL, NULL);
         /* last_rollback_loop stays at its initial -1000 on the CPU path */
-        if (c->delta < c->delta_max / 2 && c->loop - (-1000) > 1000) {
+        if (c->delta < c->delta_max / 2 && c->loop - c->last_rollback > 1000) {
             c->delta_t_val *= 0.5;
             /* CPU path: SSPRK104; GPU path with use_serk2: RK54 -> SSPRK104, else one
              * SERK2 level up (init_SERK2(2 (init - 1)) sets init + 1), capped at the
@@ -905,8 +920,152 @@ double or_loop_iteration(or_ctx* c) {
         c->rk_init = 1;
         c->delta_t_val *= 0.5;
     }
+    /* the GPU main loop's rollback (simulation_runner.cpp:280-303) */
+    if (c->rollback && c->delta > 2 * c->delta_max) {
+        const long avail = (long)c->T1 - 2;
+        const long n = avail < 0 ? 0 : (avail < 10 ? avail : 10);
+        if (n > 0) {
+            /* rollbackState(n) (simulation_control.cpp:88-113): resize every history array to
+             * T1 - n rows, delta_t = the kept last step, loop -= n, a fresh interpolation */
+            c->T1 -= (size_t)n;
+            c->delta_t_val = c->t1[c->T1 - 1] - c->t1[c->T1 - 2];
+            c->loop -= (int)n;
+            interpolate_with(c, 0, NULL, NULL);
+            c->last_rollback = c->loop;
+            t = c->t1[c->T1 - 1] + c->delta_t_val;
+        }
+        {
+            const double lo = c->delta_t_val > c->dt_min ? c->delta_t_val : c->dt_min;
+            const double hi = c->rmax[c->rk_init - 1] / c->specRad;
+            c->delta_t_val = 0.5 * (lo < hi ? lo : hi);
+        }
+        /* init_SERK2(2 (init - 3)) sets init - 1 (runge_kutta.cu:274-277); else SSPRK104 */
+        c->rk_init = c->rk_init > 3 ? c->rk_init - 1 : 2;
+    }
     return t;
 }
+
+/* ---- tail fit (tail_fit.cpp:46-265, tailFitBlendCPU) --------------------------------
+ * y_j = (1 - QK_j) / (1 - theta_j)^2 on the upper half of the last row; the constant fit
+ * of least SSE over windows of 16 (a full scan on the first call or a repeated call on the
+ * same history length, else the previous window and its two neighbours); alpha smoothed
+ * by an error-dependent gain; the blend QK -> 1 - alpha (1 - theta)^2 from the window on,
+ * ramped over 8 or 16 points, applied only when the residuals and the window SSE pass the
+ * gates.  Sums run in index order (the reference's omp simd reductions may reorder the
+ * 16-term window sums; the result differs from this order by rounding only). */
+static void tail_fit_blend(or_ctx* c) {
+    const size_t len = c->N;
+    if (len < 32) return;
+    const size_t t1len = c->T1;
+    if (t1len == 0) return;
+    const size_t base = (t1len - 1) * len;
+    const size_t start = len / 2, W = 16;
+    if (len - start < W) return;
+    const size_t Nt = len - start;
+    double* y = dalloc(Nt);
+    for (size_t j = start; j < len; ++j) {
+        const double one_m = 1.0 - c->theta[j];
+        const double denom = one_m * one_m;
+        y[j - start] = (1.0 - c->QK[base + j]) / (denom > 0 ? denom : 1e-300);
+    }
+    const int new_time_step = c->tf_prev_t1len != t1len;
+    size_t best_off = 0;
+    double best_sse = INFINITY, best_mean = 0.0;
+    size_t cand[3];
+    size_t ncand = 0;
+    if (c->tf_prev_off < 0 || !new_time_step) {
+        const size_t count = (Nt - W) / W + 1;
+        for (size_t w = 0; w < count; ++w) {
+            const size_t off = w * W;
+            double sum = 0.0, err = 0.0;
+            for (size_t k = 0; k < W; ++k) sum += y[off + k];
+            const double mean = sum / (double)W;
+            for (size_t k = 0; k < W; ++k) { const double d = y[off + k] - mean; err += d * d; }
+            if (err < best_sse) { best_sse = err; best_off = off; best_mean = mean; }
+        }
+    } else {
+        const size_t prev = (size_t)c->tf_prev_off;
+        /* candidates sorted and unique: prev - W < prev < prev + W */
+        if (prev >= W) cand[ncand++] = prev - W;
+        cand[ncand++] = prev;
+        if (prev + W + W <= Nt) cand[ncand++] = prev + W;
+        for (size_t q = 0; q < ncand; ++q) {
+            const size_t off = cand[q];
+            if (off + W > Nt) continue;
+            double sum = 0.0, err = 0.0;
+            for (size_t k = 0; k < W; ++k) sum += y[off + k];
+            const double mean = sum / (double)W;
+            for (size_t k = 0; k < W; ++k) { const double d = y[off + k] - mean; err += d * d; }
+            if (err < best_sse) { best_sse = err; best_off = off; best_mean = mean; }
+        }
+    }
+    free(y);
+    c->tf_prev_off = (long)best_off;
+    c->tf_prev_t1len = t1len;
+    const double alpha_raw = best_mean;
+    const size_t j0 = start + best_off;
+    const double var = best_sse / (double)(W - 1);
+    const double stderr_mean = sqrt((var > 0.0 ? var : 0.0) / (double)W);
+    const int rollback_detected = c->last_rollback != c->tf_known_rollback;
+    if (rollback_detected) {
+        c->tf_known_rollback = c->last_rollback;
+        if (alpha_raw != 0.0) c->tf_alpha = alpha_raw;
+    }
+    const double max_gain = 0.50, min_gain = 0.10;
+    const double rel_err = fabs(alpha_raw) > 1e-12 ? stderr_mean / fabs(alpha_raw) : 1.0;
+    double gain = max_gain / (1.0 + 2.0 * rel_err);
+    gain = gain < max_gain ? gain : max_gain;
+    gain = min_gain > gain ? min_gain : gain;
+    if (!(c->tf_alpha == c->tf_alpha) || (alpha_raw == 0.0 && rollback_detected)) {
+        if (alpha_raw != 0.0) c->tf_alpha = alpha_raw;
+        else if (!(c->tf_alpha == c->tf_alpha)) c->tf_alpha = 0.0;
+    } else if (alpha_raw == 0.0) {
+        /* keep the smoothed value */
+    } else {
+        c->tf_alpha = (1.0 - gain) * c->tf_alpha + gain * alpha_raw;
+    }
+    const double alpha = c->tf_alpha;
+    const size_t nres = len - j0;
+    double mean_abs = 0.0, mean_res = 0.0, rv = 0.0;
+    for (size_t j = j0; j < len; ++j) {
+        const double one_m = 1.0 - c->theta[j];
+        mean_abs += fabs((1.0 - alpha * one_m * one_m) - c->QK[base + j]);
+    }
+    mean_abs /= (double)nres;
+    for (size_t j = j0; j < len; ++j) {
+        const double one_m = 1.0 - c->theta[j];
+        mean_res += (1.0 - alpha * one_m * one_m) - c->QK[base + j];
+    }
+    mean_res /= (double)nres;
+    for (size_t j = j0; j < len; ++j) {
+        const double one_m = 1.0 - c->theta[j];
+        const double d = ((1.0 - alpha * one_m * one_m) - c->QK[base + j]) - mean_res;
+        rv += d * d;
+    }
+    {
+        const double den = (double)nres - 1.0;
+        rv /= (1.0 > den ? 1.0 : den);
+    }
+    const double res_std = sqrt(0.0 > rv ? 0.0 : rv);
+    const int apply = mean_abs <= 3.0 * res_std && best_sse <= sqrt(c->delta_max) && fabs(alpha) > 1e-14;
+    if (!apply) return;
+    const double ad = fabs(alpha) > 1e-12 ? fabs(alpha) : 1e-12;
+    const double alpha_change = fabs(alpha_raw - alpha) / ad;
+    const size_t ramp = alpha_change < 0.005 ? 8 : 16;
+    for (size_t j = j0; j < len; ++j) {
+        const double one_m = 1.0 - c->theta[j];
+        const double qk_fit = 1.0 - alpha * one_m * one_m;
+        double w = 1.0;
+        if (j < j0 + ramp) w = (double)(j - j0) / (double)ramp;
+        const double qk_num = c->QK[base + j];
+        c->QK[base + j] = (1.0 - w) * qk_num + w * qk_fit;
+    }
+    c->tf_applied++;
+}
+
+void or_set_tail_fit(or_ctx* c, int on) { c->tail_fit = on; }
+long or_tail_fit_applied(const or_ctx* c) { return c->tf_applied; }
+double or_tail_fit_alpha(const or_ctx* c) { return c->tf_alpha; }
 
 /* ---- setup -------------------------------------------------------------------- */
 static size_t* copy_inds(const int64_t* src, size_t n) {
@@ -945,6 +1104,9 @@ or_ctx* or_create(int64_t N_, const double* theta, const double* integ, const do
     for (int k = 0; k < 15; ++k) *f[k] = dalloc(P);
     c->rInt = dalloc(N);
     c->specRad = 4 * sqrt(fl2(c, 1));
+    c->last_rollback = -1000;
+    c->tf_prev_off = -1;
+    c->tf_known_rollback = -1000;
     ensure_cap(c, 16);
     return c;
 }

@@ -61,6 +61,13 @@ void or_serk2_coeffs(int q, double* out);
 void or_set_serk2(or_ctx* c, int on);
 /* the every-1e5-loops maintenance cadence (simulation_runner.cpp:153), lowered in tests */
 void or_set_maint_every(or_ctx* c, long every);
+/* the reference GPU main loop's rollback on an error spike (simulation_runner.cpp:280-303,
+ * simulation_control.cpp:14-76); off = the CPU path */
+void or_set_rollback(or_ctx* c, int on);
+/* the QK tail fit after every main-loop update (tail_fit.cpp:46-265); off by default */
+void or_set_tail_fit(or_ctx* c, int on);
+long or_tail_fit_applied(const or_ctx* c);
+double or_tail_fit_alpha(const or_ctx* c);
 double or_loop_iteration(or_ctx* c);
 double or_energy(or_ctx* c, double t);
 void or_sparsify(or_ctx* c, double threshold);

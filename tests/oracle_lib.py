@@ -89,6 +89,12 @@ class Oracle(_Common):
         L.or_serk2_coeffs.argtypes = [C.c_int, _dp]
         L.or_set_serk2.argtypes = [C.c_void_p, C.c_int]
         L.or_set_maint_every.argtypes = [C.c_void_p, C.c_long]
+        L.or_set_rollback.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_tail_fit.argtypes = [C.c_void_p, C.c_int]
+        L.or_tail_fit_applied.argtypes = [C.c_void_p]
+        L.or_tail_fit_applied.restype = C.c_long
+        L.or_tail_fit_alpha.argtypes = [C.c_void_p]
+        L.or_tail_fit_alpha.restype = C.c_double
         for f in ("or_destroy", "or_init_quench", "or_update", "or_loop_iteration", "or_rstep", "or_rhs",
                   "or_get_fields", "or_get_history", "or_get_control", "or_set_pos_old", "or_get_pos_old"):
             if getattr(L, f).argtypes is None:
@@ -106,6 +112,15 @@ class Oracle(_Common):
 
     def set_maint_every(self, every):
         self.L.or_set_maint_every(self.h, every)
+
+    def set_rollback(self, on):
+        self.L.or_set_rollback(self.h, 1 if on else 0)
+
+    def set_tail_fit(self, on):
+        self.L.or_set_tail_fit(self.h, 1 if on else 0)
+
+    def tail_fit_stats(self):
+        return self.L.or_tail_fit_applied(self.h), self.L.or_tail_fit_alpha(self.h)
 
     def serk2_coeffs(self, q):
         out = np.zeros(2 * q + 1)

@@ -1,10 +1,11 @@
 """The reference GPU main loop's rollback (simulation_runner.cpp:280-303, rollbackState
 simulation_control.cpp:14-76), an opt-in of dmfe_set_rollback.
 
-No CPU oracle covers it (the CPU path never rolls back), so it is pinned by what the
-reference code fixes: a step whose error exceeds 2 delta_max (forced here by a 1000x step)
-drops the last min(10, T1 - 2) rows, which must equal the earlier history bit for bit;
-delta_t becomes 0.5 min(max(t_last - t_prev, delta_t_min), rmax[method] / specRad),
+The CPU path never rolls back; the oracle restates the GPU loop's rollback
+(or_set_rollback) and the runs are compared with it.  What the reference code fixes is
+also checked directly: a step whose error exceeds 2 delta_max (forced here by a 1000x
+step) drops the last min(10, T1 - 2) rows, which must equal the earlier history bit for
+bit; delta_t becomes 0.5 min(max(t_last - t_prev, delta_t_min), rmax[method] / specRad),
 the loop counter goes back by n and the integrator restarts as SSPRK104; the context
 then continues exactly like a fresh context given the truncated state and control.
 """
@@ -12,6 +13,7 @@ import numpy as np
 import pytest
 
 import dynamite_amd as dm
+from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -81,3 +83,45 @@ def test_rollback_off_by_default():
     ctx.loop_iteration()
     assert ctx.T1 == T1 + 1 and ctx.get_control()["loop"] == c["loop"] + 1
     ctx.close()
+
+
+def test_rollback_run_matches_oracle():
+    """Free-running host loop with rollbacks on, against the oracle with its rollback on:
+    three forced error spikes (1000x steps) at different depths, each followed by 40
+    iterations.  Every iteration's proposed t, dt, loop count, integrator and T1 are
+    identical; the history is bit-identical in t1grid and rho and within 1e-10 in the
+    history arrays."""
+    g, p = setup()
+    ctx = dm.Context(g, p, rows_hint=256)
+    o = Oracle(g, p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.8)
+    dm.set_rollback(ctx, True)
+    o.set_rollback(True)
+    ctx.init_quench()
+    o.init_quench()
+    spikes = {60: 1000.0, 115: 1000.0, 170: 3000.0}
+    rollbacks = 0
+    for it in range(220):
+        if it in spikes:
+            c = ctx.get_control()
+            co = o.control()
+            assert c == co, (it, c, co)
+            ctx.set_control(spikes[it] * c["delta_t"], c["delta"], c["delta_old"], c["loop"], c["method"])
+            o.set_control(spikes[it] * co["delta_t"], co["delta"], co["delta_old"], co["loop"], co["method"])
+        T1 = ctx.T1
+        tg = ctx.loop_iteration()
+        to = o.loop_iteration()
+        c, co = ctx.get_control(), o.control()
+        rollbacks += ctx.T1 < T1
+        assert tg == to, (it, tg, to)
+        for key in ("delta_t", "loop", "method"):
+            assert c[key] == co[key], (it, key, c, co)
+        assert ctx.T1 == o.T1, it
+    assert rollbacks >= 3
+    sg, so = ctx.get_state(), o.state()
+    assert np.array_equal(sg["t1grid"], so["t1grid"]) and np.array_equal(sg["dtratio"], so["dtratio"])
+    for key in ("QK", "QR", "dQK", "dQR", "rvec", "drvec"):
+        err = np.abs(sg[key] - so[key]).max() / np.abs(so[key]).max()
+        print(f"rollback run: {key} rel err {err:.2e}")
+        assert err <= 1e-10, key
+    ctx.close()
+    o.close()
