@@ -386,6 +386,18 @@ def set_rollback(ctx, on):
     _check(lib().dmfe_set_rollback(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_rollback")
 
 
+def set_tail_fit(ctx, on):
+    """The reference main loop's QK tail fit after every step (dmfe_set_tail_fit)."""
+    _check(lib().dmfe_set_tail_fit(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_tail_fit")
+
+
+def tail_fit_stats(ctx):
+    """(blends applied, smoothed fit parameter) of the context's tail fit."""
+    n, a = C.c_int64(), C.c_double()
+    _check(lib().dmfe_tail_fit_stats(ctx._h, C.byref(n), C.byref(a)), "dmfe_tail_fit_stats")
+    return n.value, a.value
+
+
 def set_loop_graphs(ctx, on):
     """dmfe_run replays captured graphs (True, default) or launches directly (False)."""
     _check(lib().dmfe_set_loop_graphs(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_loop_graphs")

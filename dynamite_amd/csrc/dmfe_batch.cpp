@@ -170,6 +170,10 @@ int interpolate_inst(dmfe_batch* b, int i) {
     launch_stage(b->grids[(size_t)x.gi].tables(), x.H, x.W, x.M, &a, nullptr, b->stream);
     HIP_TRY(hipGetLastError());
     c.fcur = a.fbuf;
+    // a fresh interpolation: the next stage-0 RHS reads the current last row
+    if (x.W.tf_pre)
+        HIP_TRY(hipMemcpyAsync(x.W.tf_pre, x.H.V + (size_t)(c.T1 - 1) * b->grids[(size_t)x.gi].N,
+                               (size_t)b->grids[(size_t)x.gi].N * sizeof(double2), hipMemcpyDeviceToDevice, b->stream));
     return DMFE_OK;
 }
 
@@ -197,6 +201,9 @@ int enqueue_step(dmfe_batch* b, int S) {
             launch_loop_rhs(d, k, b->log_response, b->G, b->lds_rhs, b->stream);
         }
         launch_loop_stage(d, S, b->stream);
+        // the tail fit runs after update() and before the loop counter moves
+        // (simulation_runner.cpp:135-141)
+        if (d.W.tf) launch_loop_tail_fit(d, b->inst[0].prm.delta_max, b->stream);
         launch_loop_end(d, b->stream);
         if (b->obs_on) launch_batch_obs(b->d_desc, n, b->d_obs, b->obs_cap, b->stream);
         HIP_TRY(hipGetLastError());

@@ -132,6 +132,14 @@ struct dmfe_ctx {
     bool rollback = false;        // the reference GPU loop's rollback (dmfe_set_rollback)
     bool use_serk2 = false;       // the GPU loop's SERK2 promotion at the maintenance (dmfe_set_serk2)
     int64_t last_rollback = -1000;  // last_rollback_loop (simulation_runner.cpp:53)
+    // QK tail fit after every main-loop step (dmfe_set_tail_fit; tail_fit.cpp:46-265).  While
+    // it is on, W.tf_pre / W.tf point at tf_pre_buf / tf_state.  tf_fresh: W.tf_pre holds the
+    // row the last interpolation saw (a tail fit after the last step wrote it); otherwise the
+    // next step first copies the current last row there.
+    bool tail_fit = false;
+    bool tf_fresh = false;
+    double2* tf_pre_buf = nullptr;
+    TailState* tf_state = nullptr;
     bool dense_mfma = false;  // dense (B-spline) grid: A1/A2 contracted on MFMA, weights entry-major
     // forward-mode sensitivities (dmfe_tangent_init): tangent histories and work
     int tanK = 0;
@@ -311,11 +319,13 @@ int run_stage(dmfe_ctx* c, const SearchArgs* sa, const FinishArgs* f) {
 
 int run_search(dmfe_ctx* c, int mode, const double* srcB1, const double* srcB2, double* saveB1, double* saveB2) {
     const SearchArgs a = make_search(c, mode, srcB1, srcB2, saveB1, saveB2);
+    c->tf_fresh = false;  // a fresh interpolation of the current history
     return run_stage(c, &a, nullptr);
 }
 
-int run_rhs_rows(dmfe_ctx* c, double* fields) {
+int run_rhs_rows(dmfe_ctx* c, double* fields, const double2* vlast = nullptr) {
     RhsArgs a{};
+    a.vlast = vlast;
     a.T1 = (int)c->T1;
     a.tl = c->t1[c->T1 - 1];
     a.split = c->split;
@@ -350,6 +360,18 @@ int copy_out(dmfe_ctx* c, double* dst, const void* src, size_t bytes) {
 // bookkeeping (gt, t1grid, delta_t_ratio) in the reference's FP64 order.
 int rk_step_serk2(dmfe_ctx* c, int q, double* err_out);
 
+// With the tail fit on, a step's stage-0 RHS (no interpolation of its own, FSAL) reads the
+// last row from W.tf_pre: the row the previous interpolation saw.  When no tail fit wrote
+// it after the previous step, that is the current last row.
+int tail_fit_prepare(dmfe_ctx* c) {
+    if (!c->W.tf_pre) return DMFE_OK;
+    if (!c->tf_fresh && c->T1 >= 1)
+        HIP_TRY(hipMemcpyAsync(c->W.tf_pre, c->H.V + (size_t)(c->T1 - 1) * c->N, (size_t)c->N * sizeof(double2),
+                               hipMemcpyDeviceToDevice, c->stream));
+    c->tf_fresh = false;
+    return DMFE_OK;
+}
+
 int rk_step(dmfe_ctx* c, double* err_out) {
     if (c->method >= 3) return rk_step_serk2(c, 2 * (c->method - 2), err_out);
     const bool ss = (c->method != DMFE_RK54);
@@ -361,6 +383,8 @@ int rk_step(dmfe_ctx* c, double* err_out) {
         rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
         if (rc) return rc;
     }
+    rc = tail_fit_prepare(c);
+    if (rc) return rc;
     const int64_t g0row = c->T1 - 1;
     const double dt = c->dt;
     double gt[kMaxStages + 1];
@@ -387,7 +411,7 @@ int rk_step(dmfe_ctx* c, double* err_out) {
             if (rc) return rc;
             have_pending = false;
         }
-        rc = run_rhs_rows(c, nullptr);
+        rc = run_rhs_rows(c, nullptr, search ? nullptr : c->W.tf_pre);
         if (rc) return rc;
         ht[n] = 1.0;
         FinishArgs& f = pending;
@@ -522,6 +546,14 @@ int loop_iteration(dmfe_ctx* c, double* t_out) {
     int rc = rk_step(c, &err);
     if (rc) return rc;
     c->delta = err;
+    if (c->tail_fit) {
+        // after update() and before loop++ (simulation_runner.cpp:135-141)
+        launch_tail_fit(c->tables(), c->H, c->W, (int)(c->T1 - 1), (long long)c->T1, c->params.delta_max,
+                        (long long)c->last_rollback, c->stream);
+        rc = launch_check("tail_fit_kernel");
+        if (rc) return rc;
+        c->tf_fresh = true;
+    }
     c->loop++;
     const double dmax = c->params.delta_max;
     const int64_t every = c->params.maint_every > 0 ? c->params.maint_every : 100000;
@@ -610,6 +642,7 @@ void free_ctx(dmfe_ctx* c) {
     hipFree(c->W.qkb1c0); hipFree(c->W.hK); hipFree(c->W.hR); hipFree(c->W.scal);
     hipFree(c->W.fA[0]); hipFree(c->W.fA[1]);
     hipFree(c->W.sE); hipFree(c->W.sF);
+    hipFree(c->tf_pre_buf); hipFree(c->tf_state);
     tangent_free(c);
     hipFree(c->fields);
     if (c->pinned) hipHostFree(c->pinned);
@@ -878,6 +911,7 @@ extern "C" int dmfe_get_positions(dmfe_ctx* c, double* posB1, double* posB2) {
 
 extern "C" int dmfe_set_positions(dmfe_ctx* c, const double* posB1, const double* posB2) {
     if (!c || !posB1 || !posB2) return set_error(DMFE_ERR_ARG, "null argument");
+    c->tf_fresh = false;
     HIP_TRY(hipMemcpyAsync(c->W.posB1 + (size_t)c->fcur * c->N, posB1, (size_t)c->N * sizeof(double),
                            hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->W.posB2 + (size_t)c->fcur * c->P, posB2, (size_t)c->P * sizeof(double),
@@ -983,7 +1017,9 @@ extern "C" int dmfe_rstep(dmfe_ctx* c, double* r) {
 
 extern "C" int dmfe_rk_step(dmfe_ctx* c, double* err) {
     if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_rk_step: no state");
-    return rk_step(c, err);
+    const int rc = rk_step(c, err);
+    c->tf_fresh = false;  // update() alone: no tail fit follows
+    return rc;
 }
 
 extern "C" int dmfe_loop_iteration(dmfe_ctx* c, double* t_out) {
@@ -1002,6 +1038,7 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
         int rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
         if (rc) return rc;
     }
+    if (int rc = tail_fit_prepare(c)) return rc;
     const int64_t loop0 = c->loop;
     const int64_t every = c->params.maint_every > 0 ? c->params.maint_every : 100000;
     double t_prop = c->t1[c->T1 - 1];
@@ -1052,6 +1089,8 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
         c->delta = q.delta;
         c->delta_old = q.delta_old;
         c->interp_valid = true;
+        // the loop's tail fit (or its maintenance's interpolation) left W.tf_pre current
+        if (c->W.tf_pre) c->tf_fresh = true;
         t_prop = q.t_prop;
         if (rc || !c->tanK || c->loop < lim || lim >= max_loop) break;
         // a segment ended at its own limit: would the main loop go on?
@@ -1155,6 +1194,8 @@ int rk_step_serk2(dmfe_ctx* c, int q, double* err_out) {
         rc = run_search(c, kSearchB, nullptr, nullptr, nullptr, nullptr);
         if (rc) return rc;
     }
+    rc = tail_fit_prepare(c);
+    if (rc) return rc;
     const double dt = c->dt, alpha = 1.0 / (4 * q);
     const int64_t g0row = c->T1 - 1;
     const double t0 = c->t1[c->T1 - 1];
@@ -1174,7 +1215,7 @@ int rk_step_serk2(dmfe_ctx* c, int q, double* err_out) {
             if (rc) return rc;
             have_pending = false;
         }
-        rc = run_rhs_rows(c, nullptr);
+        rc = run_rhs_rows(c, nullptr, search ? nullptr : c->W.tf_pre);
         if (rc) return rc;
         FinishArgs& f = pending;
         f = FinishArgs{};
@@ -1243,6 +1284,52 @@ extern "C" int dmfe_set_serk2(dmfe_ctx* c, int on) {
 extern "C" int dmfe_set_rollback(dmfe_ctx* c, int on) {
     if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_rollback: null context");
     c->rollback = on != 0;
+    return DMFE_OK;
+}
+
+// The QK tail fit near theta -> 1 after every main-loop step (--tail-fit; tail_fit.cpp:46-265,
+// simulation_runner.cpp:135-141).  Switching it on starts the fit's state afresh (the
+// reference's function-local statics at program start).
+extern "C" int dmfe_set_tail_fit(dmfe_ctx* c, int on) {
+    if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_tail_fit: null context");
+    if (!on) {
+        c->tail_fit = false;
+        c->W.tf_pre = nullptr;
+        c->W.tf = nullptr;
+        return DMFE_OK;
+    }
+    if (c->N > kTailMaxN)
+        return set_error(DMFE_ERR_ARG, "dmfe_set_tail_fit: N = " + std::to_string(c->N) + " exceeds " +
+                                           std::to_string(kTailMaxN));
+    if (c->dense_mfma) return set_error(DMFE_ERR_STATE, "dmfe_set_tail_fit: not available on dense (B-spline) grids");
+    if (c->tanK) return set_error(DMFE_ERR_STATE, "dmfe_set_tail_fit: not available while sensitivities are carried");
+    if (!c->tf_pre_buf && hipMalloc(&c->tf_pre_buf, (size_t)c->N * sizeof(double2)) != hipSuccess)
+        return set_error(DMFE_ERR_NOMEM, "dmfe_set_tail_fit: allocation failed");
+    if (!c->tf_state && hipMalloc(&c->tf_state, sizeof(TailState)) != hipSuccess)
+        return set_error(DMFE_ERR_NOMEM, "dmfe_set_tail_fit: allocation failed");
+    TailState st{};
+    st.prev_off = -1;
+    st.prev_t1len = 0;
+    st.known_rollback = -1000;
+    st.applied = 0;
+    st.alpha = 0.0;
+    HIP_TRY(hipMemcpyAsync(c->tf_state, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->tail_fit = true;
+    c->tf_fresh = false;
+    c->W.tf_pre = c->tf_pre_buf;
+    c->W.tf = c->tf_state;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_tail_fit_stats(dmfe_ctx* c, int64_t* applied, double* alpha) {
+    if (!c) return set_error(DMFE_ERR_ARG, "dmfe_tail_fit_stats: null context");
+    if (!c->tf_state) return set_error(DMFE_ERR_STATE, "dmfe_tail_fit_stats: the tail fit was never switched on");
+    TailState st{};
+    HIP_TRY(hipMemcpyAsync(&st, c->tf_state, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (applied) *applied = st.applied;
+    if (alpha) *alpha = st.alpha;
     return DMFE_OK;
 }
 
@@ -1324,6 +1411,7 @@ extern "C" int dmfe_tangent_init(dmfe_ctx* c, int K, const double* dGamma, const
     if (c->T1 != 1) return set_error(DMFE_ERR_STATE, "dmfe_tangent_init: call right after dmfe_init_quench");
     if (c->params.log_response)
         return set_error(DMFE_ERR_STATE, "dmfe_tangent_init: the log-response interpolation is not differentiated");
+    if (c->tail_fit) return set_error(DMFE_ERR_STATE, "dmfe_tangent_init: the tail fit is not differentiated");
     tangent_free(c);
     for (int k = 0; k < K; ++k) {
         int rc = alloc_history(c->tanH[k], c->cap, c->N);

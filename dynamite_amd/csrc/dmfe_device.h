@@ -930,7 +930,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     const bool b1int = F < T1 - 1;
     const double b1rho = b1int ? H.rho[F + 1] : 1.0;
     // last rows into LDS, all loads of a thread in flight before its first store
-    const double2* Vl = H.V + (size_t)(T1 - 1) * N;
+    const double2* Vl = a.vlast ? a.vlast : H.V + (size_t)(T1 - 1) * N;
     const bool stage_tail = tail_rows && live;
     const double2* Vp = H.V + (size_t)(stage_tail ? T1 - 2 : T1 - 1) * N;
     const double2* Dl = H.D + (size_t)(T1 - 1) * N;
@@ -1058,7 +1058,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     const double in3 = dB * dB, in4 = in3 * dB;
     const double b1a = 1 - 3 * in3 - 2 * in4, b1b = 3 * in3 + 2 * in4, b1c = dB + 2 * in3 + in4, b1d = in3 + in4;
     const double2* VF0 = H.V + (size_t)(F - 1) * N;
-    const double2* VF1 = H.V + (size_t)F * N;
+    const double2* VF1 = (a.vlast && F == T1 - 1) ? a.vlast : H.V + (size_t)F * N;
     const double2* DF1 = H.D + (size_t)F * N;
     const double2* DF2 = H.D + (size_t)(b1int ? F + 1 : F) * N;
 
@@ -1424,6 +1424,148 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
     }
 }
 
+// ---- QK tail fit near theta -> 1 (tail_fit.cpp:46-265, tailFitBlendCPU) ---------------
+// After a main-loop step: y_j = (1 - QK_j) / (1 - theta_j)^2 on the upper half of the
+// last row, the constant fit of least SSE over windows of 16 (all windows on the first
+// call or a repeated call on the same history length, else the previous window and its
+// neighbours), alpha smoothed with an error-dependent gain (reset by a rollback), and
+// where the residual and SSE gates pass, QK -> 1 - alpha (1 - theta)^2 from the window
+// on with a linear ramp of 8 or 16 points.  Sums run in index order (thread 0 for the
+// cross-window ones), the reference's own plain loops; its omp simd window sums may
+// differ from this order by rounding.  One block; `pre` receives the row before the blend.
+#pragma clang fp contract(off)
+__device__ void tail_fit_body(const double* __restrict__ theta, double2* row, double2* pre, TailState* st, int N,
+                              long long t1len, double delta_max, long long rollback_loop, double* lds /* >= 2N + 96 */) {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) pre[j] = row[j];
+    const int start = N / 2, W = 16;
+    if (N < 32 || t1len == 0 || N - start < W) return;
+    const int Nt = N - start;
+    double* y = lds;             // [Nt]
+    double* qn = lds + Nt;       // [Nt] QK of the row
+    double* om = lds + 2 * Nt;   // [Nt] 1 - theta
+    double* wm = lds + 3 * Nt;   // [32] window means
+    double* ws = wm + 32;        // [32] window SSE
+    double* bc = ws + 32;        // [8] broadcast
+    for (int j = start + threadIdx.x; j < N; j += blockDim.x) {
+        const double one_m = 1.0 - theta[j];
+        const double denom = one_m * one_m;
+        const double qk = row[j].x;
+        y[j - start] = (1.0 - qk) / (denom > 0 ? denom : 1e-300);
+        qn[j - start] = qk;
+        om[j - start] = one_m;
+    }
+    const long long prev_off = st->prev_off, prev_t1len = st->prev_t1len;
+    const bool full = prev_off < 0 || prev_t1len == t1len;
+    // candidate windows in increasing offset: all of them, or prev - W, prev, prev + W
+    int ncand = 0;
+    long long cand[3] = {0, 0, 0};
+    if (full) {
+        ncand = (Nt - W) / W + 1;
+    } else {
+        if (prev_off >= W) cand[ncand++] = prev_off - W;
+        cand[ncand++] = prev_off;
+        if (prev_off + 2 * W <= Nt) cand[ncand++] = prev_off + W;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncand) {
+        const long long off = full ? (long long)threadIdx.x * W : cand[threadIdx.x];
+        double sum = 0.0, err = 0.0, mean = 0.0;
+        if (off + W <= Nt) {
+            for (int k = 0; k < W; ++k) sum += y[off + k];
+            mean = sum / (double)W;
+            for (int k = 0; k < W; ++k) {
+                const double d = y[off + k] - mean;
+                err += d * d;
+            }
+        } else {
+            err = __builtin_inf();  // skipped candidate (never chosen)
+        }
+        wm[threadIdx.x] = mean;
+        ws[threadIdx.x] = err;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long best_off = 0;
+        double best_sse = __builtin_inf(), best_mean = 0.0;
+        for (int q = 0; q < ncand; ++q) {
+            const long long off = full ? (long long)q * W : cand[q];
+            if (off + W > Nt) continue;
+            if (ws[q] < best_sse) {
+                best_sse = ws[q];
+                best_off = off;
+                best_mean = wm[q];
+            }
+        }
+        st->prev_off = best_off;
+        st->prev_t1len = t1len;
+        const double alpha_raw = best_mean;
+        const int j0 = start + (int)best_off;
+        const double var = best_sse / (double)(W - 1);
+        const double stderr_mean = sqrt((var > 0.0 ? var : 0.0) / (double)W);
+        double alpha_s = st->alpha;
+        const bool rollback_detected = rollback_loop != st->known_rollback;
+        if (rollback_detected) {
+            st->known_rollback = rollback_loop;
+            if (alpha_raw != 0.0) alpha_s = alpha_raw;
+        }
+        const double max_gain = 0.50, min_gain = 0.10;
+        const double rel_err = fabs(alpha_raw) > 1e-12 ? stderr_mean / fabs(alpha_raw) : 1.0;
+        double gain = max_gain / (1.0 + 2.0 * rel_err);
+        gain = gain < max_gain ? gain : max_gain;
+        gain = min_gain > gain ? min_gain : gain;
+        if (!(alpha_s == alpha_s) || (alpha_raw == 0.0 && rollback_detected)) {
+            if (alpha_raw != 0.0) alpha_s = alpha_raw;
+            else if (!(alpha_s == alpha_s)) alpha_s = 0.0;
+        } else if (alpha_raw != 0.0) {
+            alpha_s = (1.0 - gain) * alpha_s + gain * alpha_raw;
+        }
+        st->alpha = alpha_s;
+        const double alpha = alpha_s;
+        const int nres = N - j0;
+        double mean_abs = 0.0, mean_res = 0.0, rv = 0.0;
+        for (int j = j0; j < N; ++j) {
+            const double o = om[j - start];
+            mean_abs += fabs((1.0 - alpha * o * o) - qn[j - start]);
+        }
+        mean_abs /= (double)nres;
+        for (int j = j0; j < N; ++j) {
+            const double o = om[j - start];
+            mean_res += (1.0 - alpha * o * o) - qn[j - start];
+        }
+        mean_res /= (double)nres;
+        for (int j = j0; j < N; ++j) {
+            const double o = om[j - start];
+            const double d = ((1.0 - alpha * o * o) - qn[j - start]) - mean_res;
+            rv += d * d;
+        }
+        const double den = (double)nres - 1.0;
+        rv /= (1.0 > den ? 1.0 : den);
+        const double res_std = sqrt(0.0 > rv ? 0.0 : rv);
+        const bool apply = mean_abs <= 3.0 * res_std && best_sse <= sqrt(delta_max) && fabs(alpha) > 1e-14;
+        const double ad = fabs(alpha) > 1e-12 ? fabs(alpha) : 1e-12;
+        const double alpha_change = fabs(alpha_raw - alpha) / ad;
+        bc[0] = apply ? 1.0 : 0.0;
+        bc[1] = alpha;
+        bc[2] = (double)j0;
+        bc[3] = alpha_change < 0.005 ? 8.0 : 16.0;
+        if (apply) st->applied += 1;
+    }
+    __syncthreads();
+    if (bc[0] == 0.0) return;
+    const double alpha = bc[1];
+    const int j0 = (int)bc[2];
+    const int ramp = (int)bc[3];
+    for (int j = j0 + threadIdx.x; j < N; j += blockDim.x) {
+        const double o = om[j - start];
+        const double qk_fit = 1.0 - alpha * o * o;
+        double w = 1.0;
+        if (j < j0 + ramp) w = (double)(j - j0) / (double)ramp;
+        const double qk_num = qn[j - start];
+        row[j].x = (1.0 - w) * qk_num + w * qk_fit;
+    }
+}
+#pragma clang fp contract(on)
+
 // ---- one launch per stage boundary ------------------------------------------------
 // Block 0 (optional): finish of the previous stage.  Then (optional) the theta-row
 // search and the N phi2-row searches of the next interpolation.  The searches read
@@ -1689,6 +1831,7 @@ __device__ __forceinline__ RhsArgs step_rhs_args(const Ctl& c, const InstDesc& d
     a.log_response = logr;
     a.fields = nullptr;
     a.fbuf = (c.fc0 + searches_before(c, n + 1)) & 1;
+    a.vlast = n == 0 ? d.W.tf_pre : nullptr;  // tail fit on: the pre-blend last row
     return a;
 }
 

@@ -29,6 +29,7 @@ struct Options {
     long long len = 512;
     bool save = true, log_response = false, resume = true;
     bool serk2 = false, rollback = false;  // the reference GPU loop's behaviours (off: CPU path)
+    bool tail_fit = false;  // -F: the QK tail fit after every step (the CPU-only build never applies it)
     std::string out = "Results", grid_dir;
     long long save_every = 0, maint_every = 100000;
     int sweeps = -1;  // -w, recorded in params.txt (the CPU build always runs one sweep)
@@ -47,7 +48,8 @@ void usage() {
         "  --maint-every N   sparsify/re-interpolate cadence (default 100000)\n"
         "  --fresh           do not resume from an existing data.bin\n"
         "  -S true|false     SERK2 promotion at the maintenance (GPU-loop behaviour; default false)\n"
-        "  --rollback        roll back up to 10 steps on error spikes (GPU-loop behaviour)\n");
+        "  --rollback        roll back up to 10 steps on error spikes (GPU-loop behaviour)\n"
+        "  -F true|false     QK tail fit near theta -> 1 after every step (default false)\n");
 }
 
 bool parse(int argc, char** argv, Options& o) {
@@ -90,11 +92,8 @@ bool parse(int argc, char** argv, Options& o) {
             // version check of a resumed run
             const std::string v = val(a.c_str());
             std::fprintf(stderr, "note: option %s %s has no effect here\n", a.c_str(), v.c_str());
-        } else if (a == "-F") {
-            if (std::string(val("-F")) != "false") {
-                std::fprintf(stderr, "-F true: the tail fit (tail_fit.cpp) is not available in this build\n");
-                return false;
-            }
+        } else if (a == "-F" || a == "--tail-fit") {
+            o.tail_fit = std::string(val("-F")) != "false";
         }
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return false; }
     }
@@ -147,8 +146,8 @@ void write_params(const std::string& dir, const Options& o, dmfe_ctx* ctx, doubl
                  o.lambda, o.T0, o.Gamma);
     std::fprintf(f, "# Numerical Parameters\nlen = %lld\ntmax = %.16g\ndelta_t_min = %.16g\ndelta_max = %.16g\n", o.len,
                  o.tmax, o.dt_min, o.delta_max);
-    std::fprintf(f, "use_serk2 = %s\nlog_response_interp = %s\ntail_fit_enabled = false\nsparsify_sweeps = %d\n",
-                 o.serk2 ? "true" : "false", o.log_response ? "true" : "false", o.sweeps);
+    std::fprintf(f, "use_serk2 = %s\nlog_response_interp = %s\ntail_fit_enabled = %s\nsparsify_sweeps = %d\n",
+                 o.serk2 ? "true" : "false", o.log_response ? "true" : "false", o.tail_fit ? "true" : "false", o.sweeps);
     std::fprintf(f, "maxLoop = %lld\nrmax = [3, 13, 20, 80, 180, 320, 500, 720, 980, 1280, 1620]\n\n", o.max_loop);
     // provenance of the grid package, mirrored from its grid_params.txt as the reference
     // does (io_save_params.cpp:202-226): raw values, Tmax / alpha / delta as %.16g
@@ -238,7 +237,8 @@ int main(int argc, char** argv) {
     dmfe_ctx* ctx = nullptr;
     if (!check(dmfe_ctx_create(&g, &prm, 4096, &ctx), "dmfe_ctx_create")) return 1;
     if (!check(dmfe_set_serk2(ctx, o.serk2 ? 1 : 0), "dmfe_set_serk2") ||
-        !check(dmfe_set_rollback(ctx, o.rollback ? 1 : 0), "dmfe_set_rollback"))
+        !check(dmfe_set_rollback(ctx, o.rollback ? 1 : 0), "dmfe_set_rollback") ||
+        !check(dmfe_set_tail_fit(ctx, o.tail_fit ? 1 : 0), "dmfe_set_tail_fit"))
         return 1;
     const std::string dir = param_dir(o);
     const std::string bin = dir + "/data.bin";

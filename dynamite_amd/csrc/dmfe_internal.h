@@ -68,6 +68,18 @@ struct History {
     double* dr;   // drvec
 };
 
+// State of the QK tail fit (tail_fit.cpp:46-265) that the reference keeps in function-local
+// statics: the chosen window, the history length at the last call, the last rollback seen,
+// the smoothed fit parameter; plus the number of blends applied.
+struct TailState {
+    long long prev_off;        // -1: no window chosen yet
+    long long prev_t1len;
+    long long known_rollback;  // starts at -1000, the initial last_rollback_loop
+    long long applied;
+    double alpha;
+    double pad[3];
+};
+
 struct Work {
     double* posB1;    // [2][N]  posB1xOld, buffer fbuf of the interpolation that wrote it
     double* posB2;    // [2][P]  posB2xOld (double-buffered as Floor / diff)
@@ -84,6 +96,11 @@ struct Work {
     double2* fA[2];   // dense grids: A1, A2 fields (QK, QR) of the current RHS [P], or null
     double2* sE;      // [N] SERK2: the stage two steps back (g_e), or null
     double2* sF;      // [N] SERK2: the embedded sum of b_k g_k, or null
+    // tail fit on (else null): the last row as the step left it, before the blend (the next
+    // step's stage-0 RHS reads its fields from it, as the reference's FSAL stage uses the
+    // fields interpolated before the blend), and the fit's state
+    double2* tf_pre;  // [N]
+    TailState* tf;
 };
 
 enum SearchMode { kSearchB = 0, kSearchI = 1, kSearchSame = 2, kSearchCopy = 3 };
@@ -108,6 +125,9 @@ struct RhsArgs {
     double* fields;   // non-null: materialise the 15 fields (debug)
     int fbuf;         // Floor/diff buffer of the current interpolation
     int G;            // lanes per B2 entry (4, 8, 16 or 64; rhs_shape_valid)
+    // non-null: read the last history row (V[T1-1]) from here (the tail fit's pre-blend
+    // row at stage 0); only with N <= kTailMaxN, where last-interval B2 taps come from LDS
+    const double2* vlast;
 };
 
 // RHS row-kernel shape: G lanes per B2 entry (the reduction tree of the B2 stencil sums
@@ -308,5 +328,10 @@ void launch_loop_begin(const InstDesc& d, hipStream_t s);
 void launch_loop_end(const InstDesc& d, hipStream_t s);
 void launch_loop_stage(const InstDesc& d, int n, hipStream_t s);  // with tangents: also their finish
 void launch_loop_rhs(const InstDesc& d, int n, int log_response, int G, size_t lds, hipStream_t s);
+// tail fit of one context's device loop after the step's last stage launch
+void launch_loop_tail_fit(const InstDesc& d, double delta_max, hipStream_t s);
+// tail fit of history row `row` (t1len rows) after a host-driven step
+void launch_tail_fit(const Tables& T, const History& H, const Work& W, int row, long long t1len, double delta_max,
+                     long long rollback_loop, hipStream_t s);
 
 }  // namespace dmfe

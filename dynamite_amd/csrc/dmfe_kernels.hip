@@ -118,6 +118,25 @@ __global__ __launch_bounds__(kBeginThreads) void batch_begin_kernel(InstDesc* D,
 }
 
 // device loop of one context: the instance comes by value (no descriptor loads)
+// tail fit after a host-driven step (row = the step's last row) and after a device-loop
+// step (the row the step appended: T1s; instances not running this step are skipped)
+__global__ __launch_bounds__(kStageThreads) void tail_fit_kernel(const double* theta, double2* row, double2* pre,
+                                                                 TailState* st, int N, long long t1len, double delta_max,
+                                                                 long long rollback_loop) {
+    __shared__ double lds[3 * (kTailMaxN / 2) + 72];
+    tail_fit_body(theta, row, pre, st, N, t1len, delta_max, rollback_loop, lds);
+}
+
+__global__ __launch_bounds__(kStageThreads) void loop_tail_fit_kernel(InstDesc d, double delta_max) {
+    __shared__ double lds[3 * (kTailMaxN / 2) + 72];
+    const Ctl& c = *d.ctl;
+    const int active = c.active, T1s = c.T1s;
+    if (!active) return;
+    // the device loop has no rollbacks: last_rollback_loop keeps its initial -1000
+    tail_fit_body(d.T.theta, d.H.V + (size_t)T1s * d.T.N, d.W.tf_pre, d.W.tf, d.T.N, (long long)T1s + 1, delta_max,
+                  -1000, lds);
+}
+
 __global__ __launch_bounds__(kBeginThreads) void loop_begin_kernel(InstDesc d) {
     __shared__ int go;
     __shared__ StepHead sh;
@@ -311,6 +330,16 @@ void launch_loop_begin(const InstDesc& d, hipStream_t s) {
 
 void launch_loop_end(const InstDesc& d, hipStream_t s) {
     hipLaunchKernelGGL(loop_end_kernel, dim3(1), dim3(64), 0, s, d);
+}
+
+void launch_loop_tail_fit(const InstDesc& d, double delta_max, hipStream_t s) {
+    hipLaunchKernelGGL(loop_tail_fit_kernel, dim3(1), dim3(kStageThreads), 0, s, d, delta_max);
+}
+
+void launch_tail_fit(const Tables& T, const History& H, const Work& W, int row, long long t1len, double delta_max,
+                     long long rollback_loop, hipStream_t s) {
+    hipLaunchKernelGGL(tail_fit_kernel, dim3(1), dim3(kStageThreads), 0, s, T.theta, H.V + (size_t)row * T.N, W.tf_pre,
+                       W.tf, T.N, t1len, delta_max, rollback_loop);
 }
 
 void launch_loop_stage(const InstDesc& d, int n, hipStream_t s) {

@@ -248,6 +248,19 @@ int dmfe_set_serk2(dmfe_ctx* ctx, int on);
  * rollbacks on runs dmfe_run on the host-driven loop. */
 int dmfe_set_rollback(dmfe_ctx* ctx, int on);
 
+/* The QK tail fit of the reference's main loop (--tail-fit, tailFitBlendCPU
+ * tail_fit.cpp:46-265, called after update() at simulation_runner.cpp:135-141), off by
+ * default: after every dmfe_loop_iteration / dmfe_run step, y = (1 - QK) / (1 - theta)^2
+ * on the upper half of the last row is fitted by a constant over the 16-point window of
+ * least SSE, the fit parameter is smoothed across steps, and where the residual and SSE
+ * gates pass, QK is blended into 1 - alpha (1 - theta)^2 from that window on.  As in the
+ * reference, the next step's first RHS evaluation uses the fields interpolated before
+ * the blend.  Switching it on resets the fit's state.  N <= 1024; not with sensitivities
+ * or dense grids.  dmfe_rk_step (update() alone) does not apply it. */
+int dmfe_set_tail_fit(dmfe_ctx* ctx, int on);
+/* Blends applied since the tail fit was switched on, and the smoothed fit parameter. */
+int dmfe_tail_fit_stats(dmfe_ctx* ctx, int64_t* applied, double* alpha);
+
 /* dmfe_run replays captured hipGraphs of its 16-step launch sequence (on, default) or
  * issues the same launches directly (off).  No counterpart in the reference.  Turn it
  * off for contexts that host threads drive concurrently: a stream capture in one
