@@ -497,7 +497,7 @@ int device_loop_open(const DeviceLoop& q, dmfe_batch** out) {
     b->G = q.G;
     b->maxm = q.T.m;
     b->tanK = q.X.K;
-    b->lds_rhs = rhs_lds_bytes(N, (N + q.split - 1) / q.split, q.X.K);
+    b->lds_rhs = rhs_lds_bytes(N, rhs_chunk_cap(N, q.split), q.X.K);
     b->ept = (N + 255) / 256;
     std::vector<int2> smap{make_int2(0, -2), make_int2(0, -1)};
     for (int k = 0; k < N; ++k) smap.push_back(make_int2(0, k));
@@ -677,8 +677,7 @@ extern "C" int dmfe_batch_create(int n, const dmfe_grid* grids, const dmfe_param
         rc = alloc_work(x.W, N, rhs_max_split(N), b->stream);
         if (rc) return fail(rc);
         maxm = std::max(maxm, (int)g->m);
-        const int cw = (N + x.split - 1) / x.split;
-        b->lds_rhs = std::max(b->lds_rhs, rhs_lds_bytes(N, cw, 0));
+        b->lds_rhs = std::max(b->lds_rhs, rhs_lds_bytes(N, rhs_chunk_cap(N, x.split), 0));
         b->ept = std::max(b->ept, (N + 255) / 256);
     }
     // one lane-group width for the whole batch (its first instance's class); widened if
@@ -846,7 +845,7 @@ extern "C" int dmfe_batch_tangent_init(dmfe_batch* b, int K, const double* dGamm
         X.scal = p;
         launch_tan_init_n(x.M, X, N, b->stream);
         HIP_TRY(hipGetLastError());
-        b->lds_rhs = std::max(b->lds_rhs, rhs_lds_bytes(N, (N + x.split - 1) / x.split, K));
+        b->lds_rhs = std::max(b->lds_rhs, rhs_lds_bytes(N, rhs_chunk_cap(N, x.split), K));
     }
     b->tanK = K;
     HIP_TRY(hipStreamSynchronize(b->stream));

@@ -493,7 +493,7 @@ __device__ inline double2 stencil_last(const double* __restrict__ w, long long P
 // interpolated in log space and is not linear in the slices.
 template <bool LOGR, int G>
 __device__ __forceinline__ void b2_groups_sums(const Tables& T, const History& H, const Work& W, const RhsArgs& a, int j,
-                                          int c0, int C, const int* b2ix, const int* b2s, const double* b2u,
+                                          int chunk, int C, const int* b2ix, const int* b2s, const double* b2u,
                                           const double* b2rr, double2* b2, const double2* tail /* [3][N] or nullptr */) {
     const int N = T.N, m = T.m, T1 = a.T1;
     constexpr int GPW = kWave / G;
@@ -511,7 +511,7 @@ __device__ __forceinline__ void b2_groups_sums(const Tables& T, const History& H
         const double2* R1 = H.V + ix * N + sst;
         const double2* S1 = H.D + ix * N + sst;
         const double2* S2 = H.D + (b2int ? ix + 1 : ix) * N + sst;
-        const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
+        const double* wk = T.w[2] + ((long long)j * N + rhs_chunk_col(ee, chunk, a.split)) * m;
         double acc8[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc8[q] = 0.0;
@@ -625,10 +625,10 @@ __device__ __forceinline__ void b2_tap(const double (&c)[4], double wd, double2 
 // from the block's LDS copies `tail` = {V[T1-2], V[T1-1], D[T1-1]} when given.
 template <bool LOGR, int G, int NT>
 __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, const Work& W, const RhsArgs& a, int j,
-                                          int c0, int C, const int* b2ix, const int* b2s, const double* b2u,
+                                          int chunk, int C, const int* b2ix, const int* b2s, const double* b2u,
                                           const double* b2rr, double2* b2, const double2* tail /* [3][N] or nullptr */) {
     if constexpr (LOGR) {
-        b2_groups_sums<LOGR, G>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tail);
+        b2_groups_sums<LOGR, G>(T, H, W, a, j, chunk, C, b2ix, b2s, b2u, b2rr, b2, tail);
         return;
     }
     const int N = T.N, m = T.m, T1 = a.T1;
@@ -643,7 +643,7 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
         const size_t ix = (size_t)b2ix[ee];
         const int sst = b2s[ee];
         const bool b2int = ix < (size_t)(T1 - 1);
-        const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
+        const double* wk = T.w[2] + ((long long)j * N + rhs_chunk_col(ee, chunk, a.split)) * m;
         double c[4];
         b2_hermite(b2u[ee], b2int, b2rr[ee], c);  // linear form: b2rr holds c3
         double ak = 0.0, ar = 0.0;
@@ -700,7 +700,7 @@ __device__ __forceinline__ void b2_groups(const Tables& T, const History& H, con
 // entry's time Hermite is folded into its taps and a group closes two sums (QK, QR).
 // Per entry the arithmetic and its order are those of b2_groups.
 template <int G, int NT>
-__device__ __forceinline__ void b2_lists(const Tables& T, const History& H, const RhsArgs& a, int j, int c0, int C,
+__device__ __forceinline__ void b2_lists(const Tables& T, const History& H, const RhsArgs& a, int j, int chunk, int C,
                                          int nh, const int* lix, const int* ls, const double* lu, const int* le,
                                          double2* b2, const double2* tail /* [3][N] or nullptr */) {
     const int N = T.N, m = T.m, T1 = a.T1;
@@ -717,7 +717,7 @@ __device__ __forceinline__ void b2_lists(const Tables& T, const History& H, cons
         const int sst = ls[kk];
         const double u = lu[kk];
         const bool b2int = ix < (size_t)(T1 - 1);
-        const double* wk = T.w[2] + ((long long)j * N + c0 + e) * m;
+        const double* wk = T.w[2] + ((long long)j * N + rhs_chunk_col(e, chunk, a.split)) * m;
         const double2* R0 = H.V + (ix - 1) * N + sst;
         const double2* R1 = H.V + ix * N + sst;
         const double2* S1 = H.D + ix * N + sst;
@@ -754,7 +754,7 @@ __device__ __forceinline__ void b2_lists(const Tables& T, const History& H, cons
     for (int k = nh + (int)threadIdx.x; k < C; k += kRowThreads) {
         const int e = le[k];
         const int sst = ls[k];
-        const double* wk = T.w[2] + ((long long)j * N + c0 + e) * m;
+        const double* wk = T.w[2] + ((long long)j * N + rhs_chunk_col(e, chunk, a.split)) * m;
         double c[4];
         b2_hermite(lu[k], false, 0.0, c);
         const double2 *L0 = tail + sst, *L1 = tail + N + sst, *L2 = tail + 2 * N + sst;
@@ -776,7 +776,7 @@ __device__ __forceinline__ void b2_lists(const Tables& T, const History& H, cons
     }
 }
 
-// Row j, columns [c0, c1): the eight quadratures of QKstep/QRstep.
+// Row j, the columns of chunk `chunk` (rhs_chunk_col): the eight quadratures of QKstep/QRstep.
 //  phase 1 (B2, index_mat.cpp:18-91): one group of G lanes per target entry, lane d
 //          holding stencil tap d, so the four history-row gathers of a wave touch a
 //          few contiguous cache lines instead of one line per lane; the m-tap inner
@@ -793,7 +793,7 @@ __device__ __forceinline__ void b2_lists(const Tables& T, const History& H, cons
 // that the primal stays bit-identical to the run without tangents.
 template <int G, int K>
 __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& H, const TanArgs& X, const RhsArgs& a,
-                                                int j, int c0, int C, const int* b2ix, const int* b2s,
+                                                int j, int chunk, int C, const int* b2ix, const int* b2s,
                                                 const double* b2u, const double* b2rr, double2* b2, double2* tb2,
                                                 int cw, bool tail_rows) {
     const int N = T.N, m = T.m, T1 = a.T1;
@@ -818,7 +818,7 @@ __device__ __forceinline__ void b2_groups_multi(const Tables& T, const History& 
         const int sst = b2s[ee];
         const bool b2int = ix < (size_t)(T1 - 1);
         const size_t o0 = (ix - 1) * N + sst, o1 = ix * N + sst, o2 = (b2int ? ix + 1 : ix) * N + sst;
-        const double* wk = T.w[2] + ((long long)j * N + c0 + ee) * m;
+        const double* wk = T.w[2] + ((long long)j * N + rhs_chunk_col(ee, chunk, a.split)) * m;
         double c[4];
         b2_hermite(b2u[ee], b2int, b2rr[ee], c);  // b2rr holds c3
         double ak[1 + K], ar[1 + K];
@@ -893,8 +893,8 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
                               double* red, const TanArgs* X = nullptr) {
     const int N = T.N, m = T.m;
     const long long P = T.P;
-    const int cw = (N + a.split - 1) / a.split;
-    const int c0 = chunk * cw, c1 = min(N, c0 + cw);
+    const int cw = rhs_chunk_cap(N, a.split);
+    const int C = rhs_chunk_count(N, chunk, a.split);
     const int T1 = a.T1;
     const bool live = a.tl > 0;
     // LDS: rows {V[T1-2], V[T1-1], D[T1-1]} (only V[T1-1] when N > kTailMaxN), then the B2 arrays
@@ -910,11 +910,10 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
     int* b2s = b2ix + cw;
     // linear form (K = 0): the entry of each list slot, in the (unused) Hermite-rho array
     int* b2e = reinterpret_cast<int*>(b2rr);
-    const int C = c1 - c0;
     // the B2 operands of this thread's first entry and the scalars of phase 2 are
     // requested first, so their latency overlaps the row copies below
     const int e1 = threadIdx.x;
-    const long long k1 = (long long)j * N + c0 + e1;
+    const long long k1 = (long long)j * N + rhs_chunk_col(e1, chunk, a.split);
     double px1 = 0.0, pmax = 0.0;
     int s1 = 0;
     const double* posB2 = W.posB2 + (size_t)a.fbuf * P;  // positions of the current interpolation
@@ -989,7 +988,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
                 int ix = 1, sst = 0;
                 double u = 0.0;
                 if (e < C) {
-                    const long long k = (long long)j * N + c0 + e;
+                    const long long k = (long long)j * N + rhs_chunk_col(e, chunk, a.split);
                     const double px = e == e1 ? px1 : posB2[k];
                     size_t x = (size_t)px;
                     if (cap_hi < x) x = cap_hi;
@@ -1024,14 +1023,14 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
             }
             // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
             // instruction touch 64/G entries x a few contiguous cache lines
-            b2_lists<G, NT>(T, H, a, j, c0, C, nh, b2ix, b2s, b2u, b2e, b2, tail_rows ? rows : nullptr);
+            b2_lists<G, NT>(T, H, a, j, chunk, C, nh, b2ix, b2s, b2u, b2e, b2, tail_rows ? rows : nullptr);
         }
     } else if (live) {
         // 1a: per entry (lane per entry, coalesced): time row, stencil start, Hermite abscissa
         size_t cap_hi = (size_t)(pmax - 0.5);
         if (cap_hi > (size_t)(T1 - 1)) cap_hi = (size_t)(T1 - 1);  // memory guard
         for (int e = e1; e < C; e += kRowThreads) {
-            const long long k = (long long)j * N + c0 + e;
+            const long long k = (long long)j * N + rhs_chunk_col(e, chunk, a.split);
             const double px = e == e1 ? px1 : posB2[k];
             size_t ix = (size_t)px;
             if (cap_hi < ix) ix = cap_hi;
@@ -1049,8 +1048,8 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
         // 1b: G lanes per entry, lane d holds taps d, d+G, ...; gathers of one wave
         // instruction touch 64/G entries x a few contiguous cache lines
         if constexpr (K == 0)
-            b2_groups<LOGR, G, NT>(T, H, W, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tail_rows ? rows : nullptr);
-        else b2_groups_multi<G, K>(T, H, *X, a, j, c0, C, b2ix, b2s, b2u, b2rr, b2, tb2, cw, tail_rows);
+            b2_groups<LOGR, G, NT>(T, H, W, a, j, chunk, C, b2ix, b2s, b2u, b2rr, b2, tail_rows ? rows : nullptr);
+        else b2_groups_multi<G, K>(T, H, *X, a, j, chunk, C, b2ix, b2s, b2u, b2rr, b2, tb2, cw, tail_rows);
     }
     __syncthreads();
 
@@ -1069,7 +1068,8 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
 #pragma unroll
     for (int q = 0; q < kSums * (K > 0 ? K : 1); ++q) tacc[q] = 0.0;
 
-    for (int i = c0 + threadIdx.x; i < c1; i += kRowThreads) {
+    for (int e = threadIdx.x; e < C; e += kRowThreads) {
+        const int i = rhs_chunk_col(e, chunk, a.split);
         const long long k = (long long)j * N + i;
         double qkA1, qrA1, qkA2, qrA2, qkB1, qrB1, qkB2, qrB2;
         double2 ta1[K > 0 ? K : 1], ta2[K > 0 ? K : 1];  // tangent A1, A2 (K > 0)
@@ -1112,7 +1112,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
                 else
                     qrB1 = exp((1 - in3) * log(y0.y) + in3 * log(y1.y) - (dB + in3) * (dy1.y / y0.y));
             }
-            const double2 vb2 = b2[i - c0];
+            const double2 vb2 = b2[e];
             qkB2 = vb2.x; qrB2 = vb2.y;
         }
         // self-energies (math_sigma.cpp:7-62)
@@ -1152,7 +1152,7 @@ __device__ __forceinline__ void rhs_rows_body(const Tables& T, const History& H,
                         t[4] = (1 - in3) * y0.x - (dB + in3) * dy1.x + in3 * y1.x;
                         t[5] = (1 - in3) * y0.y - (dB + in3) * dy1.y + in3 * y1.y;
                     }
-                    const double2 tb = tb2[(size_t)h * cw + (i - c0)];
+                    const double2 tb = tb2[(size_t)h * cw + e];
                     t[6] = tb.x;
                     t[7] = tb.y;
                 }

@@ -34,6 +34,30 @@ constexpr int kMaxMethod = 11;  // integrators 1 RK54, 2 SSPRK104, 3.. SERK2 lev
 // beside V[T1-1], so B2 targets in the last time interval read the block's LDS copy.
 constexpr int kTailMaxN = 1024;
 
+// Columns of RHS block (row j, chunk c of `split`): segments of kColSeg columns dealt to the
+// chunks round-robin (segment s to chunk s mod split), so that the B2 entries of a row
+// whose time positions lie inside the history (a contiguous column range, the costly
+// gathers) are shared by all of its blocks.  Entry e of a chunk is column
+// (e / kColSeg * split + c) * kColSeg + e % kColSeg; a segment is one wavefront's worth of
+// contiguous columns, so lane-per-entry reads stay coalesced.
+constexpr int kColSeg = 64;
+__host__ __device__ inline int rhs_chunk_col(int e, int chunk, int split) {
+    return ((e / kColSeg) * split + chunk) * kColSeg + e % kColSeg;
+}
+// entries of chunk c (columns < N)
+__host__ __device__ inline int rhs_chunk_count(int N, int chunk, int split) {
+    const int nseg = (N + kColSeg - 1) / kColSeg;
+    if (chunk >= nseg) return 0;
+    const int segs = (nseg - chunk + split - 1) / split;
+    const int last = (chunk + (segs - 1) * split == nseg - 1) ? nseg * kColSeg - N : 0;  // short final segment
+    return segs * kColSeg - last;
+}
+// the largest chunk (LDS sizing)
+__host__ __device__ inline int rhs_chunk_cap(int N, int split) {
+    const int nseg = (N + kColSeg - 1) / kColSeg;
+    return ((nseg + split - 1) / split) * kColSeg;
+}
+
 // dynamic LDS of one RHS row block: [N] V[T1-1], ([N] V[T1-2], [N] D[T1-1] if N <= kTailMaxN),
 // [cw] B2 values, [cw] Hermite abscissae, [cw] rho of the entries' next rows, [cw] time
 // rows, [cw] stencil starts, 8 bytes of alignment, and for K tangent directions [K N]

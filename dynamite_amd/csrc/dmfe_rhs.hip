@@ -7,7 +7,7 @@ namespace dmfe {
 void launch_tan_rhs(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
                     const TanArgs& X, hipStream_t s) {
     const dim3 grid((unsigned)(T.N * a.split));
-    const int cw = (T.N + a.split - 1) / a.split;
+    const int cw = rhs_chunk_cap(T.N, a.split);
     const size_t lds = rhs_lds_bytes(T.N, cw, X.K);
 #define DMFE_TR(GG, KK) \
     hipLaunchKernelGGL((rhs_rows_tan_kernel<GG, KK>), grid, dim3(kRowThreads), lds, s, T, H, W, M, a, X)
@@ -27,7 +27,7 @@ void launch_tan_rhs(const Tables& T, const History& H, const Work& W, const Mode
 void launch_rhs_rows(const Tables& T, const History& H, const Work& W, const Model& M, const RhsArgs& a,
                      hipStream_t s) {
     const dim3 grid((unsigned)(T.N * a.split));
-    const int cw = (T.N + a.split - 1) / a.split;
+    const int cw = rhs_chunk_cap(T.N, a.split);
     const size_t lds = rhs_lds_bytes(T.N, cw, 0);
 #define DMFE_RHS(LOG, G, NT) \
     hipLaunchKernelGGL((rhs_rows_kernel<LOG, G, NT>), grid, dim3(kRowThreads), lds, s, T, H, W, M, a)
