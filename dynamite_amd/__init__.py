@@ -391,6 +391,11 @@ def set_tail_fit(ctx, on):
     _check(lib().dmfe_set_tail_fit(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_tail_fit")
 
 
+def set_gpu_variants(ctx, on):
+    """The reference GPU path's interpolation: bisection only, per-element B1 clamp."""
+    _check(lib().dmfe_set_gpu_variants(ctx._h, C.c_int(1 if on else 0)), "dmfe_set_gpu_variants")
+
+
 def tail_fit_stats(ctx):
     """(blends applied, smoothed fit parameter) of the context's tail fit."""
     n, a = C.c_int64(), C.c_double()

@@ -74,6 +74,7 @@ struct Inst {
     History H{};
     Work W{};
     int split = 1;
+    int gpu_variants = 0;  // the reference GPU path's interpolation (a context's device loop)
     int64_t cap = 0;
     TanArgs X{};  // tangents (dmfe_batch_tangent_init, or a context's device loop); X.K = 0: none
     History tH[kMaxTan] = {};  // the tangent histories this batch owns (dmfe_batch_tangent_init)
@@ -123,6 +124,7 @@ int upload_desc(dmfe_batch* b) {
         d[i].M = x.M;
         d[i].ctl = b->d_ctl + i;
         d[i].split = x.split;
+        d[i].gpu_variants = x.gpu_variants;
         d[i].X = x.X;
     }
     HIP_TRY(hipMemcpyAsync(b->d_desc, d.data(), d.size() * sizeof(InstDesc), hipMemcpyHostToDevice, b->stream));
@@ -542,6 +544,7 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
     }
     Inst& x = b->inst[0];
     x.gi = 0; x.prm = q.prm; x.M = q.M; x.H = q.H; x.W = q.W; x.cap = q.cap; x.X = q.X;
+    x.gpu_variants = q.gpu_variants;
     const bool rec = corr && corr_cap > 0;
     if (rec != b->obs_on) b->obs_on = rec;  // build_graph re-captures when the recording flag changes
     Ctl& c = b->ctl[0];

@@ -136,6 +136,7 @@ struct dmfe_ctx {
     // it is on, W.tf_pre / W.tf point at tf_pre_buf / tf_state.  tf_fresh: W.tf_pre holds the
     // row the last interpolation saw (a tail fit after the last step wrote it); otherwise the
     // next step first copies the current last row there.
+    bool gpu_variants = false;  // the reference GPU path's interpolation (dmfe_set_gpu_variants)
     bool tail_fit = false;
     bool tf_fresh = false;
     double2* tf_pre_buf = nullptr;
@@ -301,6 +302,14 @@ SearchArgs make_search(const dmfe_ctx* c, int mode, const double* srcB1, const d
     a.tl = c->t1[c->T1 - 1];
     a.srcB1 = srcB1; a.srcB2 = srcB2; a.saveB1 = saveB1; a.saveB2 = saveB2;
     a.fbuf = 1 - c->fcur;
+    if (c->gpu_variants) {
+        // the reference GPU path bisects at every interpolation (interpolateGPU ignores its
+        // input positions and `same`, interpolation_core.cu:65-72) and clamps the B1 row
+        // per element (:23-42)
+        a.mode = kSearchB;
+        a.srcB1 = a.srcB2 = nullptr;
+        a.gpu_clamp = 1;
+    }
     return a;
 }
 
@@ -1076,6 +1085,7 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
         q.delta = c->delta;
         q.delta_old = c->delta_old;
         q.t_prop = t_prop;
+        q.gpu_variants = c->gpu_variants ? 1 : 0;
         const int64_t done = c->loop - loop0;
         double* cseg = (corr && done < corr_cap) ? corr + 3 * done : nullptr;
         rc = device_loop_run(&c->dloop, q, tmax, lim, cseg, cseg ? corr_cap - done : 0, nullptr);
@@ -1319,6 +1329,13 @@ extern "C" int dmfe_set_tail_fit(dmfe_ctx* c, int on) {
     c->tf_fresh = false;
     c->W.tf_pre = c->tf_pre_buf;
     c->W.tf = c->tf_state;
+    return DMFE_OK;
+}
+
+extern "C" int dmfe_set_gpu_variants(dmfe_ctx* c, int on) {
+    if (!c) return set_error(DMFE_ERR_ARG, "dmfe_set_gpu_variants: null context");
+    c->gpu_variants = on != 0;
+    c->interp_valid = false;  // the next step starts from a fresh interpolation
     return DMFE_OK;
 }
 

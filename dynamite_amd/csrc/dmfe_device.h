@@ -360,7 +360,12 @@ __device__ __forceinline__ void search_theta_row(const Tables& T, const Work& W,
     const size_t t1len = (size_t)a.T1;
     for (int i = threadIdx.x; i < N; i += kStageThreads) {
         size_t f = (size_t)floor(pos[i]);
-        if (f < 1) f = 1;
+        if (a.gpu_clamp) {
+            // diffNfloorKernel (interpolation_core.cu:23-42): each element's own ceil
+            const size_t own = (size_t)ceil(pos[i]);
+            if (own - 1 < f) f = own - 1;
+            if (f < 1) f = 1;
+        } else if (f < 1) f = 1;
         else if (f > maxCeil) f = maxCeil;
         if (f > t1len - 1) f = t1len > 1 ? t1len - 1 : 1;  // memory guard; never binds on valid positions
         W.Floor[(size_t)a.fbuf * N + i] = (int)f;
@@ -1722,6 +1727,13 @@ __device__ inline bool make_search_dev(const C& c, const InstDesc& d, int n, Sea
         }
     }
     a.fbuf = (c.fc0 + searches_before(c, n) + 1) & 1;
+    if (d.gpu_variants) {
+        // the reference GPU path bisects at every interpolation (interpolateGPU ignores its
+        // input positions and `same`, interpolation_core.cu:65-72)
+        a.mode = kSearchB;
+        a.srcB1 = a.srcB2 = nullptr;
+        a.gpu_clamp = 1;
+    }
     return true;
 }
 

@@ -139,6 +139,7 @@ struct SearchArgs {
     double* saveB1;       // optional second destination
     double* saveB2;
     int fbuf;             // Floor/diff buffer written
+    int gpu_clamp;        // B1 time row clamped per element to [1, ceil(p) - 1] (the reference GPU path)
 };
 
 struct RhsArgs {
@@ -296,7 +297,7 @@ struct InstDesc {
     Model M;
     Ctl* ctl;
     int split;
-    int pad;
+    int gpu_variants;  // the reference GPU path's interpolation (dmfe_set_gpu_variants)
     TanArgs X;  // forward-mode tangents carried by the instance (X.K = 0: none)
 };
 
@@ -332,6 +333,7 @@ struct DeviceLoop {
     int method, fcur;
     double dt, delta, delta_old;
     double t_prop;  // in: the loop variable t (t1grid.back() at a fresh call); out: last proposed t
+    int gpu_variants;  // the reference GPU path's interpolation variants
 };
 // *slot caches the loop's launch maps and captured graphs between calls (NULL at first;
 // released with device_loop_close).

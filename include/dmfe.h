@@ -258,6 +258,16 @@ int dmfe_set_rollback(dmfe_ctx* ctx, int on);
  * the blend.  Switching it on resets the fit's state.  N <= 1024; not with sensitivities
  * or dense grids.  dmfe_rk_step (update() alone) does not apply it. */
 int dmfe_set_tail_fit(dmfe_ctx* ctx, int on);
+/* The interpolation of the reference's GPU path (interpolateGPU, interpolation_core.cu),
+ * off by default (the CPU path is the contract): every interpolation by bisection
+ * (bsearchPosSortedGPU; the interpolation search seeded by the previous positions, the
+ * SSPRK104 position reuse and RK54's `same` are not used there) and the B1 time row
+ * clamped per element to [1, ceil(p) - 1] (diffNfloorKernel, :23-42) instead of
+ * [1, ceil(max p) - 1].  The clamp changes no value on finite histories (at integer
+ * positions both forms reduce to the same history row exactly); bisection and the
+ * interpolation search bracket identically but evaluate the position with different
+ * formulas, so runs differ from the default in the last bits. */
+int dmfe_set_gpu_variants(dmfe_ctx* ctx, int on);
 /* Blends applied since the tail fit was switched on, and the smoothed fit parameter. */
 int dmfe_tail_fit_stats(dmfe_ctx* ctx, int64_t* applied, double* alpha);
 

@@ -48,6 +48,10 @@ struct or_ctx {
     /* QK tail fit near theta -> 1 (tail_fit.cpp:46-265, --tail-fit): on/off and the state the
      * reference keeps in function-local statics (prev_best_off, prev_t1len, alpha_smoothed,
      * last_known_rollback), here per context */
+    /* the reference GPU path's interpolation (interpolation_core.cu:23-42,65-72): every
+     * interpolation by bisection (its input positions and `same` ignored) and the B1 time
+     * row clamped per element to [1, ceil(p) - 1] instead of [1, ceil(max p) - 1] */
+    int gpu_variants;
     int tail_fit;
     long tf_prev_off; /* -1: SIZE_MAX sentinel, no window chosen yet */
     size_t tf_prev_t1len;
@@ -355,6 +359,7 @@ static void fill(double* x, size_t n, double v) { for (size_t i = 0; i < n; ++i)
  * mode 2: reuse the given positions ("same")                                      */
 static void interpolate_with(or_ctx* c, int mode, const double* inB1, const double* inB2) {
     const size_t N = c->N, P = N * N, T1 = c->T1;
+    if (c->gpu_variants) mode = 0;
     const double tl = c->t1[T1 - 1];
     double* posB1 = dalloc(N);
     double* posB2 = dalloc(P);
@@ -392,7 +397,11 @@ static void interpolate_with(or_ctx* c, int mode, const double* inB1, const doub
     double* diff = dalloc(N);
     for (size_t i = 0; i < N; ++i) {
         size_t f = (size_t)floor(posB1[i]);
-        if (f < 1) f = 1;
+        if (c->gpu_variants) {
+            const size_t own = (size_t)ceil(posB1[i]);  /* diffNfloorKernel: own ceil per element */
+            if (own - 1 < f) f = own - 1;
+            if (f < 1) f = 1;
+        } else if (f < 1) f = 1;
         else if (f > maxCeil) f = maxCeil;
         F[i] = f;
         diff[i] = (double)F[i] - posB1[i];
@@ -824,6 +833,17 @@ void or_set_serk2(or_ctx* c, int on) { c->use_serk2 = on; }
 void or_set_maint_every(or_ctx* c, long every) { c->maint_every = every; }
 void or_set_rollback(or_ctx* c, int on) { c->rollback = on; }
 
+/* rollbackState(n) (simulation_control.cpp:88-113): resize every history array to T1 - n
+ * rows, delta_t = the kept last step, loop -= n, a fresh interpolation; 0 when n >= T1 - 1 */
+int or_rollback_state(or_ctx* c, int n) {
+    if (n < 0 || (size_t)n >= c->T1 - 1) return 0;
+    c->T1 -= (size_t)n;
+    c->delta_t_val = c->t1[c->T1 - 1] - c->t1[c->T1 - 2];
+    c->loop -= n;
+    interpolate_with(c, 0, NULL, NULL);
+    return 1;
+}
+
 /* ---- sparsify_utils.cpp:14-97 (CPU-only build) ------------------------------------ */
 void or_sparsify(or_ctx* c, double threshold) {
     const size_t N = c->N, T1 = c->T1;
@@ -925,12 +945,7 @@ double or_loop_iteration(or_ctx* c) {
         const long avail = (long)c->T1 - 2;
         const long n = avail < 0 ? 0 : (avail < 10 ? avail : 10);
         if (n > 0) {
-            /* rollbackState(n) (simulation_control.cpp:88-113): resize every history array to
-             * T1 - n rows, delta_t = the kept last step, loop -= n, a fresh interpolation */
-            c->T1 -= (size_t)n;
-            c->delta_t_val = c->t1[c->T1 - 1] - c->t1[c->T1 - 2];
-            c->loop -= (int)n;
-            interpolate_with(c, 0, NULL, NULL);
+            or_rollback_state(c, (int)n);
             c->last_rollback = c->loop;
             t = c->t1[c->T1 - 1] + c->delta_t_val;
         }
@@ -1064,6 +1079,7 @@ static void tail_fit_blend(or_ctx* c) {
 }
 
 void or_set_tail_fit(or_ctx* c, int on) { c->tail_fit = on; }
+void or_set_gpu_variants(or_ctx* c, int on) { c->gpu_variants = on; }
 long or_tail_fit_applied(const or_ctx* c) { return c->tf_applied; }
 double or_tail_fit_alpha(const or_ctx* c) { return c->tf_alpha; }
 

@@ -64,8 +64,11 @@ void or_set_maint_every(or_ctx* c, long every);
 /* the reference GPU main loop's rollback on an error spike (simulation_runner.cpp:280-303,
  * simulation_control.cpp:14-76); off = the CPU path */
 void or_set_rollback(or_ctx* c, int on);
+int or_rollback_state(or_ctx* c, int n);
 /* the QK tail fit after every main-loop update (tail_fit.cpp:46-265); off by default */
 void or_set_tail_fit(or_ctx* c, int on);
+/* the reference GPU path's interpolation variants (bisection only, per-element B1 clamp) */
+void or_set_gpu_variants(or_ctx* c, int on);
 long or_tail_fit_applied(const or_ctx* c);
 double or_tail_fit_alpha(const or_ctx* c);
 double or_loop_iteration(or_ctx* c);

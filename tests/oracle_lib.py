@@ -91,6 +91,7 @@ class Oracle(_Common):
         L.or_set_maint_every.argtypes = [C.c_void_p, C.c_long]
         L.or_set_rollback.argtypes = [C.c_void_p, C.c_int]
         L.or_set_tail_fit.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_gpu_variants.argtypes = [C.c_void_p, C.c_int]
         L.or_tail_fit_applied.argtypes = [C.c_void_p]
         L.or_tail_fit_applied.restype = C.c_long
         L.or_tail_fit_alpha.argtypes = [C.c_void_p]
@@ -118,6 +119,9 @@ class Oracle(_Common):
 
     def set_tail_fit(self, on):
         self.L.or_set_tail_fit(self.h, 1 if on else 0)
+
+    def set_gpu_variants(self, on):
+        self.L.or_set_gpu_variants(self.h, 1 if on else 0)
 
     def tail_fit_stats(self):
         return self.L.or_tail_fit_applied(self.h), self.L.or_tail_fit_alpha(self.h)

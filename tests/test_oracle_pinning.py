@@ -235,3 +235,43 @@ def test_quench_invariants():
     assert np.abs(QR[:, -1] - 1).max() <= 1e-12
     assert np.all(np.diff(E) <= 0)
     assert np.all(np.diff(QK[:, 0]) <= 0)  # QK(t, 0) decays for T0 = inf
+
+
+def test_gpu_path_b1_clamp_changes_no_value():
+    """The reference GPU path clamps each B1 time row to [1, ceil(p) - 1] on its own
+    (interpolation_core.cu:23-42) where the CPU path uses [1, ceil(max p) - 1]
+    (interpolation_core.cpp:58-80).  They differ only at integer positions p = k < max p,
+    where the CPU form takes row k with d = 0 and the GPU form row k - 1 with d = -1; the
+    Hermite coefficients are then exactly (1, 0, 0, 0) and (0, 1, 0, 0), so both give
+    history row k - 1 bit for bit.  Checked on a grid with uniform theta and a history at
+    integer times, where every B1 position is an integer."""
+    import copy
+
+    N = 33
+    g = copy.copy(dm.generate_grid(N, 10.0, 5, dm.INTERP_POLY, 9, 0))
+    g.theta = np.arange(N, dtype=np.float64) / 32.0
+    T = N  # t1grid = 0, 1, ..., 32: theta_i * t = i exactly
+    rng = np.random.default_rng(5)
+    t1 = np.arange(T, dtype=np.float64)
+    QK = 1 + 0.1 * rng.standard_normal((T, N))
+    QR = 1 + 0.1 * rng.standard_normal((T, N))
+    dQK = 0.01 * rng.standard_normal((T, N))
+    dQR = 0.01 * rng.standard_normal((T, N))
+    r = 1 + rng.random(T)
+    dr = 0.1 * rng.standard_normal(T)
+    rho = np.ones(T)
+    rho[:2] = 0.0
+    out = []
+    for variant in (False, True):
+        o = Oracle(g)
+        o.set_history(t1, rho, r, dr, QK.ravel(), QR.ravel(), dQK.ravel(), dQR.ravel())
+        o.set_gpu_variants(variant)
+        o.interpolate(0)
+        pos = o.positions()[0]
+        assert np.array_equal(pos, np.arange(1, N + 1, dtype=np.float64))  # all integer
+        out.append(o.fields())
+        o.close()
+    for k in ("QKB1", "QRB1", "SigmaKB1", "SigmaRB1", "rInt"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+    # and they are the history rows themselves: B1 target (j, i) at position j + 1 -> row j
+    assert np.array_equal(out[0]["QKB1"].reshape(N, N), QK[:N, :])
