@@ -406,6 +406,7 @@ void free_batch(dmfe_batch* b) {
         free_hist(x.H);
         Work& W = x.W;
         for (void* p : {(void*)W.posB1, (void*)W.posB2, (void*)W.saveB1, (void*)W.saveB2, (void*)W.Floor, (void*)W.diff,
+                        (void*)W.rpk,
                         (void*)W.rInt, (void*)W.partial, (void*)W.qkb1c0, (void*)W.hK, (void*)W.hR, (void*)W.scal})
             (void)hipFree(p);
     }
@@ -450,6 +451,7 @@ int alloc_work(Work& W, int N, int split, hipStream_t stream) {
     HIP_TRY(hipMalloc(&W.saveB2, 3 * bP));
     HIP_TRY(hipMalloc(&W.Floor, 2 * (size_t)N * sizeof(int)));
     HIP_TRY(hipMalloc(&W.diff, 2 * bN));
+    HIP_TRY(hipMalloc(&W.rpk, 2 * (size_t)N * sizeof(double4)));
     HIP_TRY(hipMalloc(&W.rInt, bN));
     HIP_TRY(hipMalloc(&W.partial, bN * (size_t)split * kSums));
     HIP_TRY(hipMalloc(&W.qkb1c0, bN));

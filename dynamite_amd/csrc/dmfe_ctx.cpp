@@ -647,7 +647,7 @@ void free_ctx(dmfe_ctx* c) {
     for (int f = 0; f < 3; ++f) { hipFree(c->inds[f]); hipFree(c->w[f]); }
     free_history(c->H);
     hipFree(c->W.posB1); hipFree(c->W.posB2); hipFree(c->W.saveB1); hipFree(c->W.saveB2);
-    hipFree(c->W.Floor); hipFree(c->W.diff); hipFree(c->W.rInt); hipFree(c->W.partial);
+    hipFree(c->W.Floor); hipFree(c->W.diff); hipFree(c->W.rpk); hipFree(c->W.rInt); hipFree(c->W.partial);
     hipFree(c->W.qkb1c0); hipFree(c->W.hK); hipFree(c->W.hR); hipFree(c->W.scal);
     hipFree(c->W.fA[0]); hipFree(c->W.fA[1]);
     hipFree(c->W.sE); hipFree(c->W.sF);
@@ -727,6 +727,7 @@ extern "C" int dmfe_ctx_create(const dmfe_grid* g, const dmfe_params* prm, int64
     ok = ok && hipMalloc(&c->W.posB1, 2 * bN) == hipSuccess && hipMalloc(&c->W.posB2, 2 * bP) == hipSuccess &&
          hipMalloc(&c->W.saveB1, 3 * bN) == hipSuccess && hipMalloc(&c->W.saveB2, 3 * bP) == hipSuccess &&
          hipMalloc(&c->W.Floor, 2 * (size_t)N * sizeof(int)) == hipSuccess && hipMalloc(&c->W.diff, 2 * bN) == hipSuccess &&
+         hipMalloc(&c->W.rpk, 2 * (size_t)N * sizeof(double4)) == hipSuccess &&
          hipMalloc(&c->W.rInt, bN) == hipSuccess &&
          hipMalloc(&c->W.partial, bN * (size_t)rhs_max_split(N) * kSums) == hipSuccess &&
          hipMalloc(&c->W.qkb1c0, bN) == hipSuccess && hipMalloc(&c->W.hK, bN * kMaxStages) == hipSuccess &&

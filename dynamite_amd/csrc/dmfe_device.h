@@ -241,6 +241,34 @@ __device__ inline size_t bracket_of(double pos, size_t L) {
 // reproduces the sequential walk exactly.  The first sweep takes its brackets
 // from the previous positions (init), which are usually already exact, so most
 // rows finish in one or two sweeps.
+// rInt (index_vec.cpp:138-156) in two parts.  The theta-row search, which writes the B1
+// prologue (Floor, diff) of an interpolation, evaluates the terms over history rows older
+// than the interpolation's newest row; the finish of the previous stage may be writing
+// that row's r, dr, rho in the same launch.  The finish block that needs rInt adds the
+// newest row's terms (rint_finish), so it reads no history row through Floor.  The
+// operations and their order are those of rint_at:
+//   kind 0: every row older -> the value itself;
+//   kind 1: f + 1 is the newest row -> P - ca dr[f+1] / rho[f+1];
+//   kind 2: f is the newest row (last interval) -> (P + ca r[f]) - cb dr[f].
+#pragma clang fp contract(off)
+__device__ inline double4 rint_terms(const History& H, int f, double d, int T1, bool live) {
+    if (!live) return make_double4(H.r[0], 0.0, 0.0, 0.0);
+    const double d2 = d * d, d3 = d * (d * d);
+    if (f < T1 - 1) {
+        const double p = (1 - 3 * d2 - 2 * d3) * H.r[f - 1] + (3 * d2 + 2 * d3) * H.r[f] - (d + 2 * d2 + d3) * H.dr[f];
+        if (f + 1 < T1 - 1) return make_double4(p - (d2 + d3) * H.dr[f + 1] / H.rho[f + 1], 0.0, 0.0, 0.0);
+        return make_double4(p, d2 + d3, 0.0, 1.0);
+    }
+    return make_double4((1 - d2) * H.r[f - 1], d2, d + d2, 2.0);
+}
+// rw, drw, rhow: r, dr, rho of the interpolation's newest history row
+__device__ inline double rint_finish(double4 q, double rw, double drw, double rhow) {
+    if (q.w == 0.0) return q.x;
+    if (q.w == 1.0) return q.x - q.y * drw / rhow;
+    return (q.x + q.y * rw) - q.z * drw;
+}
+#pragma clang fp contract(on)
+
 template <int EPT>
 __device__ __forceinline__ void search_row(const ListView& list, double tl, const double* __restrict__ elem, const double* init,
                            double* out, double* out2, int len, int mode, double* lds) {
@@ -332,7 +360,8 @@ __device__ __forceinline__ void search_phi2_row(const Tables& T, const Work& W, 
 // B1 positions of the theta row and the B1 prologue Floor / diff
 // (interpolation_core.cpp:22-24, 57-84).
 template <int EPT>
-__device__ __forceinline__ void search_theta_row(const Tables& T, const Work& W, const SearchArgs& a, double* lds, double* red) {
+__device__ __forceinline__ void search_theta_row(const Tables& T, const History& H, const Work& W, const SearchArgs& a,
+                                                 double* lds, double* red) {
     const int N = T.N;
     double* pos = W.posB1 + (size_t)a.fbuf * N;  // double buffered as the B2 positions
     const double* old = W.posB1 + (size_t)(1 - a.fbuf) * N;
@@ -368,8 +397,10 @@ __device__ __forceinline__ void search_theta_row(const Tables& T, const Work& W,
         } else if (f < 1) f = 1;
         else if (f > maxCeil) f = maxCeil;
         if (f > t1len - 1) f = t1len > 1 ? t1len - 1 : 1;  // memory guard; never binds on valid positions
+        const double d = (double)f - pos[i];
         W.Floor[(size_t)a.fbuf * N + i] = (int)f;
-        W.diff[(size_t)a.fbuf * N + i] = (double)f - pos[i];
+        W.diff[(size_t)a.fbuf * N + i] = d;
+        W.rpk[(size_t)a.fbuf * N + i] = rint_terms(H, (int)f, d, a.T1, a.tl > 0);
     }
 }
 
@@ -438,6 +469,7 @@ __device__ inline double rint_at(const History& H, const int* Floor, const doubl
                (d2 + d3) * H.dr[f + 1] / H.rho[f + 1];
     return (1 - d2) * H.r[f - 1] + d2 * H.r[f] - (d + d2) * H.dr[f];
 }
+
 
 // taps whose weights one lane requests in one memory round trip (the stencil width of the
 // default poly order 9)
@@ -1234,10 +1266,10 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
     const double t = a.t_rhs;
     const double2* last = H.V + (size_t)a.lr * N;
     const double qK0 = last[0].x;
-    // rInt (index_vec.cpp:138-156) from the B1 prologue of the current interpolation
-    const bool live = t > 0;
-    const int* Fl = W.Floor + (size_t)a.fbuf * N;
-    const double* Df = W.diff + (size_t)a.fbuf * N;
+    // rInt (index_vec.cpp:138-156): the current interpolation's terms over the older rows
+    // plus those of its newest row a.lr (rint_finish)
+    const double4* Rq = W.rpk + (size_t)a.fbuf * N;
+    const double rw = H.r[a.lr], drw = H.dr[a.lr], rhow = H.rho[a.lr];
     // operands of the stage combination (earlier stages' RHS, the step's start row) do not
     // depend on this stage's RHS: they are requested first, so that their latency
     // overlaps the RHS epilogue and the reductions below
@@ -1268,7 +1300,7 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
             }
         }
     }
-    const double s1 = fl1(M, qK0) / M.T0, s2 = -rint_at(H, Fl, Df, N - 1, a.lr + 1, live), s3 = qK0 / M.T0,
+    const double s1 = fl1(M, qK0) / M.T0, s2 = -rint_finish(Rq[N - 1], rw, drw, rhow), s3 = qK0 / M.T0,
                  s4 = 2 * M.Gamma;
     double hk[EPT], hr[EPT];
 #pragma unroll
@@ -1286,7 +1318,7 @@ __device__ __forceinline__ void finish_block(const Tables& T, const History& H, 
         }
         const double th = T.theta[j];
         const double fA = t * th, fR = t * (1 - th);
-        const double ri = rint_at(H, Fl, Df, j, a.lr + 1, live);
+        const double ri = rint_finish(Rq[j], rw, drw, rhow);
         W.rInt[j] = ri;
         const double qK = last[j].x, qR = last[j].y, b1 = W.qkb1c0[j];
         double d1 = b1 * s1 + qK * s2;
@@ -1594,7 +1626,7 @@ __global__ __launch_bounds__(kStageThreads) void stage_kernel(Tables T, History 
     }
     if (flags & kHasTheta) {
         if (b == 0) {
-            search_theta_row<EPT>(T, W, sa, lds, red);
+            search_theta_row<EPT>(T, H, W, sa, lds, red);
             return;
         }
         --b;
@@ -1811,7 +1843,7 @@ __device__ __forceinline__ void stage_role_body(const InstDesc& d, int role, int
     const int active = c.active, S = c.S;
     const SearchArgs a = c.srch[n];
     if (!active || n > S || a.mode < 0) return;
-    if (role == -1) search_theta_row<EPT>(d.T, d.W, a, lds, red);
+    if (role == -1) search_theta_row<EPT>(d.T, d.H, d.W, a, lds, red);
     else search_phi2_row<EPT>(d.T, d.W, a, role, lds);
 }
 

@@ -111,6 +111,7 @@ struct Work {
     double* saveB2;   // [3][P]
     int* Floor;       // [2][N]  B1 time rows (double-buffered: a launch's search writes
     double* diff;     // [2][N]  one buffer while its finish block reads the other)
+    double4* rpk;     // [2][N]  rInt terms of the older history rows (rint_terms), buffered as Floor
     double* rInt;     // [N]
     double* partial;  // [N][split][kSums]
     double* qkb1c0;   // [N]  QKB1[j, 0]
