@@ -32,10 +32,15 @@
 #include "grid/interpolation_weights.hpp"
 #include "grid/grid_io.hpp"
 #include "io/io_utils.hpp"
+#include "simulation/simulation_control.hpp"
+#include "simulation/tail_fit.hpp"
 
 SimulationConfig config;
 SimulationData* sim = nullptr;
 RKData* rk = nullptr;
+// defined by simulation_runner.cpp (not compiled here); read by the tail fit
+int last_rollback_loop = -1000;
+static bool g_tail_fit = false;
 
 namespace {
 template <class T>
@@ -215,6 +220,9 @@ double ref_loop_iteration(void) {
     double t = sim->h_t1grid.back() + config.delta_t;
     config.delta_old = config.delta;
     config.delta = update();
+    // --tail-fit of a CUDA build run with --gpu false (simulation_runner.cpp:135-141); the
+    // CPU-only build never calls it
+    if (g_tail_fit) tailFitBlendCPU();
     config.loop++;
     if (config.loop % 100000 == 0) {
         size_t prev_size = sim->h_t1grid.size();
@@ -245,6 +253,14 @@ double ref_loop_iteration(void) {
     }
     return t;
 }
+
+// The tail fit after every ref_loop_iteration (tailFitBlendCPU, tail_fit.cpp:46-265).  Its
+// state lives in the reference's function-local statics: one run per loaded library.
+void ref_set_tail_fit(int on) { g_tail_fit = on != 0; }
+
+// rollbackState(n) (simulation_control.cpp:14-124, CPU-only build): truncate the history
+// by n rows, delta_t from the kept last step, loop -= n, interpolate().  1 on success.
+int ref_rollback_state(int n) { return rollbackState(n) ? 1 : 0; }
 
 // ---- grid generator pieces (src/grid/*) -------------------------------------
 static std::vector<long double> g_theta, g_phi1, g_phi2, g_int;
