@@ -660,7 +660,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
+        # no dataset: the state is the quench's own initial condition integrated --prep steps
+        "data": "synthetic (quench from its initial condition, then the timed steps on that live state)",
         "config": {"workload": "plain quench, L=512 poly-9 grid (spline 5, Tmax 1e5), p=3 q=4 lambda=0.5 "
                                "Gamma=0.6 T0=inf; adaptive RK steps (dmfe_run) on the live state after the prep steps",
                    "L": N, "stencil_width": g.m, "prep_steps": a.prep, "T1_end": T1,
