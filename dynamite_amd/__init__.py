@@ -338,6 +338,17 @@ class Context:
         self._tanK = 2
         return grad, t.value, n.value
 
+    def adjoint(self, seeds, tmax, max_loop):
+        """dmfe_adjoint: (seeds . dE,QK0/dGamma, seeds . dE,QK0/dlambda) of a quench run."""
+        sd = np.ascontiguousarray(seeds, np.float64)
+        if sd.size != 2:
+            raise ValueError("adjoint: two seeds (energy, QK(t,0))")
+        dG, dL, t, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
+        _check(lib().dmfe_adjoint(self._h, _ptr(sd), C.c_double(tmax), C.c_int64(max_loop), C.byref(dG), C.byref(dL),
+                                  C.byref(t), C.byref(n)), "dmfe_adjoint")
+        self._tanK = 2
+        return dG.value, dL.value, t.value, n.value
+
     def synchronize(self):
         _check(lib().dmfe_synchronize(self._h), "dmfe_synchronize")
 

@@ -1482,6 +1482,20 @@ extern "C" int dmfe_tangent_state(dmfe_ctx* c, int k, double* dQK, double* dQR, 
     return DMFE_OK;
 }
 
+extern "C" int dmfe_adjoint(dmfe_ctx* c, const double* seeds, double tmax, int64_t max_loop, double* dGamma,
+                            double* dLambda, double* t_final, int64_t* steps) {
+    if (!c || !seeds || !dGamma || !dLambda) return set_error(DMFE_ERR_ARG, "dmfe_adjoint: null argument");
+    if (!std::isfinite(seeds[0]) || !std::isfinite(seeds[1]))
+        return set_error(DMFE_ERR_ARG, "dmfe_adjoint: seeds must be finite");
+    double g[4];
+    const int rc = dmfe_gradients(c, tmax, max_loop, g, t_final, steps);
+    if (rc) return rc;
+    // seeds^T J with J = [[dE/dGamma, dE/dlambda], [dQK0/dGamma, dQK0/dlambda]]
+    *dGamma = seeds[0] * g[0] + seeds[1] * g[2];
+    *dLambda = seeds[0] * g[1] + seeds[1] * g[3];
+    return DMFE_OK;
+}
+
 extern "C" int dmfe_gradients(dmfe_ctx* c, double tmax, int64_t max_loop, double* grad, double* t_final,
                               int64_t* steps) {
     if (!c || !grad) return set_error(DMFE_ERR_ARG, "dmfe_gradients: null argument");

@@ -310,6 +310,15 @@ int dmfe_tangent_state(dmfe_ctx* ctx, int k, double* dQK, double* dQR, double* d
 /* Quench + run (as dmfe_run) carrying both directions; grad = [dE/dGamma, dE/dlambda,
  * dQK0/dGamma, dQK0/dlambda] at the final proposed time *t_final. */
 int dmfe_gradients(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* grad, double* t_final, int64_t* steps);
+/* The adjoint entry point of the drop-in boundary (dmfe_adjoint(ctx, seeds, dT, dlambda)):
+ * the vector-Jacobian product seeds . d(E, QK(t_final,0))/d(Gamma, lambda) of a quench run
+ * to tmax / max_loop: *dGamma = seeds[0] dE/dGamma + seeds[1] dQK0/dGamma, *dLambda the
+ * same in lambda.  With two parameters the product is formed from the forward sweep of
+ * dmfe_gradients (two tangent directions through the same run, the dt sequence frozen),
+ * which costs less than a reverse sweep with checkpoint recomputation; the result is the
+ * same derivative of the discrete run.  t_final, steps may be NULL. */
+int dmfe_adjoint(dmfe_ctx* ctx, const double* seeds, double tmax, int64_t max_loop, double* dGamma, double* dLambda,
+                 double* t_final, int64_t* steps);
 /* dmfe_gradients for n (grid, params) points at once (no counterpart in the reference,
  * which runs one parameter point per process): each point gets its own context, stream
  * and host thread (at most 16 at a time), so their kernels overlap on the device.

@@ -138,23 +138,41 @@ def test_tangent_is_linear_in_the_seed(nominal):
 
 
 class QuenchObservables(torch.autograd.Function):
-    """(Gamma, lambda) -> (E, QK(t,0)) of the quench run; backward from the tangents."""
+    """(Gamma, lambda) -> (E, QK(t,0)) of the quench run; backward through dmfe_adjoint
+    with the upstream gradient as its seeds."""
 
     @staticmethod
     def forward(ctx_, G, lam):
         c = dm.Context(grid(), params(float(G), float(lam)))
-        grad, t, n = c.gradients(1e300, STEPS)
-        E, q0 = c.observables(t)
+        c.init_quench()
+        _, corr = c.run(1e300, STEPS, corr_cap=STEPS)
+        E, q0 = c.observables(corr[-1][0])  # at the last proposed time
         c.close()
-        ctx_.save_for_backward(torch.tensor(grad, dtype=torch.float64))
+        ctx_.save_for_backward(G, lam)
         return torch.tensor([E, q0], dtype=torch.float64)
 
     @staticmethod
     def backward(ctx_, g):
-        (grad,) = ctx_.saved_tensors
-        J = grad.reshape(2, 2)  # rows E, QK0; columns Gamma, lambda
-        v = g @ J
-        return v[0], v[1]
+        G, lam = ctx_.saved_tensors
+        c = dm.Context(grid(), params(float(G), float(lam)))
+        dG, dL, _, _ = c.adjoint(g.detach().cpu().numpy(), 1e300, STEPS)
+        c.close()
+        return torch.tensor(dG, dtype=torch.float64), torch.tensor(dL, dtype=torch.float64)
+
+
+def test_adjoint_is_the_seeded_jacobian(nominal):
+    """dmfe_adjoint(seeds) = seeds . J, J from dmfe_gradients on the same run."""
+    c = dm.Context(grid(), params())
+    grad, t, n = c.gradients(1e300, STEPS)
+    c.close()
+    J = grad.reshape(2, 2)
+    c = dm.Context(grid(), params())
+    dG, dL, t2, n2 = c.adjoint([1.0, 3.0], 1e300, STEPS)
+    c.close()
+    assert (t2, n2) == (t, n)
+    assert dG == 1.0 * J[0, 0] + 3.0 * J[1, 0] and dL == 1.0 * J[0, 1] + 3.0 * J[1, 1]
+    with pytest.raises(dm.DmfeError):
+        dm.Context(grid(), params()).adjoint([np.nan, 1.0], 1e300, STEPS)
 
 
 def test_torch_autograd_binding(nominal):
