@@ -17,9 +17,20 @@
 #include <string>
 #include <sys/stat.h>
 #include <time.h>
+#include <algorithm>
+#include <csignal>
 #include <vector>
 
 namespace {
+
+// SIGINT / SIGTERM / SIGHUP end the run after the current segment: the final save runs
+// and the exit status is 130 (simulation_text_ui.cpp:30-65, simulation_runner.cpp:126,467)
+volatile std::sig_atomic_t g_interrupted = 0;
+void on_signal(int) { g_interrupted = 1; }
+
+// main-loop iterations per dmfe_run call: the correlation rows are written and the signal
+// flag is checked between calls
+constexpr int64_t kSegment = 256;
 
 struct Options {
     int p = 3, p2 = 12;
@@ -266,6 +277,9 @@ int main(int argc, char** argv) {
             return 1;
         }
     }
+    std::signal(SIGINT, on_signal);
+    std::signal(SIGTERM, on_signal);
+    std::signal(SIGHUP, on_signal);
     struct timespec t0s;
     clock_gettime(CLOCK_MONOTONIC, &t0s);
     std::vector<double> t1;
@@ -277,25 +291,44 @@ int main(int argc, char** argv) {
     double dt = 0, delta = 0, delta_old = 0;
     int64_t loop = 0;
     int method = 0;
+    bool aborted = false;
+    // rollbacks move the loop counter back, so a call may run more iterations than it was
+    // asked to advance the counter: with them on, one loop per call (at most 11 iterations)
+    const int64_t seg = o.rollback ? 1 : kSegment;
+    const int64_t cap = kSegment + 16;
+    std::vector<double> rec(3 * cap);
     const double eps = 2.220446049250313e-16;
+    // runSimulation's while-loop (simulation_runner.cpp:125) in dmfe_run calls of at most seg
+    // iterations, each ending at the next checkpoint
     for (;;) {
+        if (g_interrupted) {
+            aborted = true;
+            break;
+        }
         dmfe_get_control(ctx, &dt, &delta, &delta_old, &loop, &method);
         if (!(t < o.tmax && loop < o.max_loop && dt >= o.dt_min - eps * std::fmax(std::fabs(dt), std::fabs(o.dt_min))))
             break;
-        if (!check(dmfe_loop_iteration(ctx, &t), "dmfe_loop_iteration")) return 1;
-        if (!check(dmfe_observables(ctx, t, &E, &qk0), "dmfe_observables")) return 1;
-        if (corr) std::fprintf(corr, "%.14f\t%.14f\t%.14f\n", t, E, qk0);
-        if (o.save && o.save_every > 0 && (loop + 1) % o.save_every == 0) {
+        int64_t lim = std::min<int64_t>(o.max_loop, loop + seg);
+        if (o.save && o.save_every > 0) lim = std::min<int64_t>(lim, (loop / o.save_every + 1) * o.save_every);
+        int64_t n = 0;
+        if (!check(dmfe_run(ctx, o.tmax, lim, rec.data(), cap, &n), "dmfe_run")) return 1;
+        for (int64_t k = 0; k < n && k < cap; ++k) {
+            t = rec[3 * k]; E = rec[3 * k + 1]; qk0 = rec[3 * k + 2];
+            if (corr) std::fprintf(corr, "%.14f\t%.14f\t%.14f\n", t, E, qk0);
+        }
+        dmfe_get_control(ctx, &dt, &delta, &delta_old, &loop, &method);
+        if (o.save && o.save_every > 0 && n > 0 && loop % o.save_every == 0) {
             std::fflush(corr);
             if (!check(dmfe_save_state(ctx, bin.c_str()), "dmfe_save_state")) return 1;
         }
+        if (n == 0) break;
     }
     struct timespec t1s;
     clock_gettime(CLOCK_MONOTONIC, &t1s);
     const double elapsed = (t1s.tv_sec - t0s.tv_sec) + 1e-9 * (t1s.tv_nsec - t0s.tv_nsec);
     dmfe_get_control(ctx, &dt, &delta, &delta_old, &loop, &method);
-    std::printf("loop %lld  t %.14f  E %.14f  QK(t,0) %.14f  dt %.6g  %.2f loops/s\n", (long long)loop, t, E, qk0, dt,
-                elapsed > 0 ? loop / elapsed : 0.0);
+    std::printf("%sloop %lld  t %.14f  E %.14f  QK(t,0) %.14f  dt %.6g  %.2f loops/s\n",
+                aborted ? "interrupted: " : "", (long long)loop, t, E, qk0, dt, elapsed > 0 ? loop / elapsed : 0.0);
     if (o.save) {
         std::fclose(corr);
         if (!check(dmfe_save_state(ctx, bin.c_str()), "dmfe_save_state")) return 1;
@@ -307,5 +340,5 @@ int main(int argc, char** argv) {
         write_params(dir, o, ctx, t1.back(), Elast, elapsed);
     }
     dmfe_ctx_destroy(ctx);
-    return 0;
+    return aborted ? 130 : 0;
 }

@@ -147,3 +147,42 @@ def test_driver_run_and_resume(tmp_path):
     h, _ = parse_data_bin(os.path.join(pdir, "data.bin"))
     assert h["loop"] == 50
     ctx.close()
+
+
+def test_driver_interrupt_saves_and_exits_130(tmp_path):
+    """SIGINT ends the run after the current segment: correlation rows up to there, the final
+    data.bin of that state, exit status 130 (simulation_text_ui.cpp:30-65,
+    simulation_runner.cpp:126,467).  The signal comes from the host; every kernel exits."""
+    import signal
+    import time
+
+    g = grid()
+    gdir = str(tmp_path / "Grid_data" / str(N))
+    g.write(gdir)
+    out = str(tmp_path / "Results")
+    args = [DRIVER, "-p", "3", "-q", "4", "-l", "0.5", "-G", "0.8", "-T", "inf", "-L", str(N), "-t", "1e9",
+            "-m", "100000000", "--grid", gdir, "-o", out]
+    pdir = os.path.join(out, "p=3_q=4_lambda=0.50_T0=inf_G=0.80_L=64")
+    cfile = os.path.join(pdir, "correlation.txt")
+    print("driver run to be interrupted", flush=True)
+    p = subprocess.Popen(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    try:
+        t0 = time.time()
+        while time.time() - t0 < 60:  # until the run has written a few segments
+            if p.poll() is not None:
+                break
+            if os.path.exists(cfile) and os.path.getsize(cfile) > 2000 * 60:
+                break
+            time.sleep(0.2)
+        assert p.poll() is None, "the run ended before the signal: " + p.stderr.read()
+        p.send_signal(signal.SIGINT)
+        stdout, stderr = p.communicate(timeout=60)
+    finally:
+        if p.poll() is None:
+            p.kill()
+    assert p.returncode == 130, (p.returncode, stderr)
+    assert "interrupted" in stdout
+    h, arrs = parse_data_bin(os.path.join(pdir, "data.bin"))
+    corr = np.loadtxt(cfile)
+    assert h["loop"] > 0 and corr.shape == (int(h["loop"]), 3)
+    assert abs(corr[-1, 0] - arrs["t1grid"][-1]) <= 1e-12 * max(1.0, abs(corr[-1, 0]))  # last proposed t (14 decimals)
