@@ -105,6 +105,10 @@ struct dmfe_batch {
     bool graph_s0[2] = {false, false};  // the graph holds the stage-0 search launch
     int64_t steps_run = 0;
     bool borrowed = false;  // device_loop_run: tables, instance buffers and stream belong to a context
+    // device loop: the correlation record of the last rollback iteration, recomputed after
+    // the rollback (t, E, QK(t,0)); rb_pending until device_loop_run stores it
+    bool rb_pending = false;
+    double rb_rec[3] = {0.0, 0.0, 0.0};
     InstDesc desc0{};       // device loop: instance 0's descriptor, passed by value to its kernels
 };
 
@@ -300,6 +304,48 @@ int sparsify_inst(dmfe_batch* b, int i, double threshold) {
     return DMFE_OK;
 }
 
+// The rollback of the GPU main loop (simulation_runner.cpp:280-303, rollbackState
+// simulation_control.cpp:14-76) after an iteration whose error norm exceeded 2 delta_max:
+// drop the last n = min(10, T1 - 2) rows, delta_t = the kept last step, loop -= n, a fresh
+// interpolation; the iteration's correlation record is that of the truncated history at
+// t1grid.back() + delta_t; then delta_t = 0.5 min(max(delta_t, dt_min), rmax / specRad)
+// and the integrator restarts as SSPRK104 (SERK2 levels above the first step down one).
+int rollback_inst(dmfe_batch* b, int i) {
+    Inst& x = b->inst[(size_t)i];
+    Ctl& c = b->ctl[(size_t)i];
+    const dmfe_params& p = x.prm;
+    const int N = b->grids[(size_t)x.gi].N;
+    const int64_t n = std::min<int64_t>(10, std::max<int64_t>(0, (int64_t)c.T1 - 2));
+    if (n > 0) {
+        c.T1 -= (int)n;
+        double tl[2] = {0.0, 0.0};
+        HIP_TRY(hipMemcpyAsync(tl, x.H.t1 + (c.T1 - 2), 2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        c.dt = tl[1] - tl[0];
+        c.loop -= n;
+        int rc = interpolate_inst(b, i);
+        if (rc) return rc;
+        c.last_rollback = c.loop;
+        c.t_prop = tl[1] + c.dt;
+        // the record of this iteration (simulation_runner.cpp:305-341 after the rollback)
+        launch_row_scalar(b->grids[(size_t)x.gi].tables(), x.H, x.W, x.M, c.T1 - 1, c.t_prop, 1, b->stream);
+        HIP_TRY(hipGetLastError());
+        double e[2] = {0.0, 0.0};
+        HIP_TRY(hipMemcpyAsync(e, x.W.scal + 2, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipMemcpyAsync(e + 1, &x.H.V[(size_t)(c.T1 - 1) * N].x, sizeof(double), hipMemcpyDeviceToHost,
+                               b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        b->rb_pending = true;
+        b->rb_rec[0] = c.t_prop;
+        b->rb_rec[1] = e[0];
+        b->rb_rec[2] = e[1];
+    }
+    c.dt = 0.5 * std::min(std::max(c.dt, p.delta_t_min), p.rmax[c.method - 1] / c.specRad);
+    c.method = c.method > 3 ? c.method - 1 : DMFE_SSPRK104;
+    c.status = kRunning;
+    return DMFE_OK;
+}
+
 // the rest of the main-loop body after loop % cadence == 0 (simulation_runner.cpp:161-278)
 int maintenance(dmfe_batch* b, int i) {
     Inst& x = b->inst[(size_t)i];
@@ -314,7 +360,8 @@ int maintenance(dmfe_batch* b, int i) {
     }
     int rc = interpolate_inst(b, i);
     if (rc) return rc;
-    if (c.delta < dmax / 2 && c.loop - (-1000) > 1000) {
+    // last_rollback_loop stays -1000 unless rollbacks are on (simulation_runner.cpp:53,220)
+    if (c.delta < dmax / 2 && c.loop - c.last_rollback > 1000) {
         c.dt *= 0.5;
         c.method = DMFE_SSPRK104;
     }
@@ -333,6 +380,7 @@ int maintenance(dmfe_batch* b, int i) {
         c.dt *= 0.5;
     }
     c.status = kRunning;
+    if (c.rollback && c.delta > 2 * dmax) return rollback_inst(b, i);
     return DMFE_OK;
 }
 
@@ -372,6 +420,7 @@ int service(dmfe_batch* b, bool* changed) {
                                                  "(size the arena for the run; tangents do not cross a sparsification)");
         if (c.status == kNeedMaint) rc = maintenance(b, i);
         else if (c.status == kNeedRows) rc = grow_inst(b, i);
+        else if (c.status == kNeedRollback) rc = rollback_inst(b, i);
         else continue;
         if (rc) return rc;
         *changed = true;
@@ -569,9 +618,13 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
     c.delta_old = q.delta_old;
     c.t_prop = q.t_prop;
     c.active = 0;
+    c.rollback = q.rollback ? 1 : 0;
+    c.last_rollback = q.last_rollback;
     int rc = upload_desc(b);
     if (!rc) rc = push_ctl(b);
-    const int64_t loop0 = q.loop;
+    // iterations run: the loop counter's advance, except that a rollback moves it back
+    int64_t iters = 0;
+    b->rb_pending = false;
     std::vector<double> ring(rec ? (size_t)kLoopChunk * 4 : 0);
     while (!rc) {
         const int S = max_stages(b);
@@ -600,8 +653,10 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
                 rc = set_error(DMFE_ERR_HIP, std::string("dmfe_run: records: ") + hipGetErrorString(e));
                 break;
             }
+            // the ring is indexed by loop; within a replay the loop counter only grows (an
+            // instance that needs a rollback stops for the rest of the replay)
             for (int64_t lp = before + 1; lp <= b->ctl[0].loop; ++lp) {
-                const int64_t k = lp - loop0 - 1;  // row of corr
+                const int64_t k = iters + (lp - before) - 1;  // row of corr: the iteration
                 if (k >= corr_cap) break;
                 const double* o = ring.data() + (size_t)((lp - 1) % kLoopChunk) * 4;
                 corr[3 * k + 0] = o[0];
@@ -609,8 +664,14 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
                 corr[3 * k + 2] = o[2];
             }
         }
+        iters += b->ctl[0].loop - before;
         bool changed = false;
         rc = service(b, &changed);
+        if (b->rb_pending) {  // the rollback iteration's record, after the rollback
+            b->rb_pending = false;
+            if (rec && iters >= 1 && iters - 1 < corr_cap)
+                for (int v = 0; v < 3; ++v) corr[3 * (iters - 1) + v] = b->rb_rec[v];
+        }
     }
     // the caller adopts the (possibly replaced) arena and the control state
     q.H = x.H;
@@ -619,7 +680,8 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
     q.T1 = f.T1; q.loop = f.loop; q.method = f.method; q.fcur = f.fcur;
     q.dt = f.dt; q.delta = f.delta; q.delta_old = f.delta_old;
     q.t_prop = f.t_prop;
-    if (steps) *steps = f.loop - loop0;
+    q.last_rollback = f.last_rollback;
+    if (steps) *steps = iters;
     return rc;
 }
 

@@ -1049,7 +1049,7 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
         if (rc) return rc;
     }
     if (int rc = tail_fit_prepare(c)) return rc;
-    const int64_t loop0 = c->loop;
+    int64_t done = 0;  // iterations (a rollback moves the loop counter back)
     const int64_t every = c->params.maint_every > 0 ? c->params.maint_every : 100000;
     double t_prop = c->t1[c->T1 - 1];
     int rc = DMFE_OK;
@@ -1087,9 +1087,13 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
         q.delta_old = c->delta_old;
         q.t_prop = t_prop;
         q.gpu_variants = c->gpu_variants ? 1 : 0;
-        const int64_t done = c->loop - loop0;
+        q.rollback = c->rollback ? 1 : 0;
+        q.last_rollback = c->last_rollback;
         double* cseg = (corr && done < corr_cap) ? corr + 3 * done : nullptr;
-        rc = device_loop_run(&c->dloop, q, tmax, lim, cseg, cseg ? corr_cap - done : 0, nullptr);
+        int64_t seg_iters = 0;
+        rc = device_loop_run(&c->dloop, q, tmax, lim, cseg, cseg ? corr_cap - done : 0, &seg_iters);
+        done += seg_iters;
+        c->last_rollback = q.last_rollback;
         c->H = q.H;
         c->cap = q.cap;
         c->T1 = q.T1;
@@ -1115,7 +1119,7 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
             break;
         }
     }
-    if (steps) *steps = c->loop - loop0;
+    if (steps) *steps = done;
     if (t_out) *t_out = t_prop;
     // host mirrors of t1grid and delta_t_ratio
     c->t1.resize((size_t)c->T1);
@@ -1359,8 +1363,9 @@ extern "C" int dmfe_set_loop_graphs(dmfe_ctx* c, int on) {
 
 extern "C" int dmfe_run(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps) {
     if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_run: no state");
-    // dense MFMA contexts, per-launch profiling, rollbacks and SERK2 keep the host-driven loop
-    if (!c->dense_mfma && !c->profiling && !c->rollback && !c->use_serk2 && c->method < 3)
+    // dense MFMA contexts, per-launch profiling, SERK2 and rollbacks with tangents keep the
+    // host-driven loop
+    if (!c->dense_mfma && !c->profiling && !c->use_serk2 && c->method < 3 && !(c->rollback && c->tanK))
         return run_device_loop(c, tmax, max_loop, corr, corr_cap, steps);
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];

@@ -1775,7 +1775,7 @@ __device__ __forceinline__ void end_body(const InstDesc& d) {
     Ctl& c = *d.ctl;
     // every operand is loaded before the first test and the first store (one memory
     // round trip); rho of the new last row is the value its finish wrote (Ctl::rho_step)
-    const int active = c.active, T1s = c.T1s, fc0 = c.fc0, S = c.S, method = c.method;
+    const int active = c.active, T1s = c.T1s, fc0 = c.fc0, S = c.S, method = c.method, rollback = c.rollback;
     const long long loop = c.loop + 1, maint_every = c.maint_every;
     const double delta = d.W.scal[1], delta_old = c.delta_old, dmax = c.delta_max, specRad = c.specRad;
     const double dt_min = c.dt_min, rho_last = c.rho_step;
@@ -1808,6 +1808,9 @@ __device__ __forceinline__ void end_body(const InstDesc& d) {
         dt *= 0.5;
     }
     c.dt = dt;
+    // the rollback (simulation_runner.cpp:280-303) truncates the history and interpolates
+    // afresh: the host does it between graph replays
+    if (rollback && delta > 2 * dmax) c.status = kNeedRollback;
 }
 
 __global__ void batch_end_kernel(InstDesc* D, int n) {

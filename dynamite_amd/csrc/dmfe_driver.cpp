@@ -292,10 +292,10 @@ int main(int argc, char** argv) {
     int64_t loop = 0;
     int method = 0;
     bool aborted = false;
-    // rollbacks move the loop counter back, so a call may run more iterations than it was
-    // asked to advance the counter: with them on, one loop per call (at most 11 iterations)
-    const int64_t seg = o.rollback ? 1 : kSegment;
-    const int64_t cap = kSegment + 16;
+    // rollbacks move the loop counter back by up to 10, so a call may run up to 11 times
+    // as many iterations as it advances the counter
+    const int64_t seg = kSegment;
+    const int64_t cap = o.rollback ? 11 * kSegment + 16 : kSegment + 16;
     std::vector<double> rec(3 * cap);
     const double eps = 2.220446049250313e-16;
     // runSimulation's while-loop (simulation_runner.cpp:125) in dmfe_run calls of at most seg

@@ -259,7 +259,7 @@ int scratch_address(double** out);
 inline size_t rhs_scratch_doubles(int N, int split) { return (size_t)N * split * kSums + 4 * (size_t)N; }
 
 // ---- batched quenches with device-side step control ----------------------------------
-enum CtlStatus { kRunning = 0, kDone = 1, kNeedMaint = 2, kNeedRows = 3 };
+enum CtlStatus { kRunning = 0, kDone = 1, kNeedMaint = 2, kNeedRows = 3, kNeedRollback = 4 };
 
 // Per-instance control state (the SimulationConfig / RKData scalars of one quench),
 // owned by the device while a batch runs.
@@ -272,6 +272,9 @@ struct Ctl {
     long long loop;
     double dt, delta, delta_old;
     double t_prop;      // proposed t of the last iteration (simulation_runner.cpp:130)
+    int rollback;       // the GPU main loop's rollback (simulation_runner.cpp:280-303) is on
+    int pad_rb;
+    long long last_rollback = -1000;  // last_rollback_loop (simulation_runner.cpp:53), -1000 until one
     // limits
     double tmax;
     long long max_loop;
@@ -335,6 +338,8 @@ struct DeviceLoop {
     double dt, delta, delta_old;
     double t_prop;  // in: the loop variable t (t1grid.back() at a fresh call); out: last proposed t
     int gpu_variants;  // the reference GPU path's interpolation variants
+    int rollback;      // rollbacks on error spikes (simulation_runner.cpp:280-303)
+    int64_t last_rollback;  // in/out: last_rollback_loop
 };
 // *slot caches the loop's launch maps and captured graphs between calls (NULL at first;
 // released with device_loop_close).

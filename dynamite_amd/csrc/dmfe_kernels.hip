@@ -131,10 +131,10 @@ __global__ __launch_bounds__(kStageThreads) void loop_tail_fit_kernel(InstDesc d
     __shared__ double lds[3 * (kTailMaxN / 2) + 72];
     const Ctl& c = *d.ctl;
     const int active = c.active, T1s = c.T1s;
+    const long long last_rollback = c.last_rollback;
     if (!active) return;
-    // the device loop has no rollbacks: last_rollback_loop keeps its initial -1000
     tail_fit_body(d.T.theta, d.H.V + (size_t)T1s * d.T.N, d.W.tf_pre, d.W.tf, d.T.N, (long long)T1s + 1, delta_max,
-                  -1000, lds);
+                  last_rollback, lds);
 }
 
 __global__ __launch_bounds__(kBeginThreads) void loop_begin_kernel(InstDesc d) {

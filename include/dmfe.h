@@ -225,7 +225,9 @@ int dmfe_loop_iteration(dmfe_ctx* ctx, double* t_out);
  * of iterations run.  The loop runs with its step control on the device (stage
  * arguments, error norm and dt rule evaluated by kernels, 16 steps per host round
  * trip); results are bit-identical to repeated dmfe_loop_iteration calls.  Contexts
- * on dense grids and contexts with per-launch profiling on use the host loop. */
+ * on dense grids, with per-launch profiling or SERK2 on use the host loop.  Rows of corr
+ * are iterations: after a rollback the loop counter repeats values, as correlation.txt
+ * does. */
 int dmfe_run(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps);
 
 /* SERK2, the reference GPU loop's stabilized second-order-Chebyshev integrator
@@ -244,8 +246,9 @@ int dmfe_set_serk2(dmfe_ctx* ctx, int on);
  * n = min(10, T1 - 2) steps are dropped (history truncated, delta_t = the kept last
  * step, loop -= n, re-interpolation), then delta_t = 0.5 min(max(delta_t,
  * delta_t_min), rmax[method] / specRad) and the integrator restarts as SSPRK104; the
- * maintenance's dt halving then waits 1000 loops after a rollback.  A context with
- * rollbacks on runs dmfe_run on the host-driven loop. */
+ * maintenance's dt halving then waits 1000 loops after a rollback.  dmfe_run takes
+ * the rollback on its device-driven loop (the host truncates between graph replays), on
+ * the host-driven loop when tangents are carried. */
 int dmfe_set_rollback(dmfe_ctx* ctx, int on);
 
 /* The QK tail fit of the reference's main loop (--tail-fit, tailFitBlendCPU

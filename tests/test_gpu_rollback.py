@@ -125,3 +125,58 @@ def test_rollback_run_matches_oracle():
         assert err <= 1e-10, key
     ctx.close()
     o.close()
+
+
+def host_records(ctx, iters):
+    rows = []
+    for _ in range(iters):
+        t = ctx.loop_iteration()
+        E, q0 = ctx.observables(t)
+        rows.append((t, E, q0))
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("tail_fit", [False, True])
+def test_device_loop_rollback_matches_host_loop(tail_fit):
+    """dmfe_run with rollbacks on (the host truncates between the device loop's graph
+    replays) against the host-driven loop: two forced error spikes, correlation rows per
+    iteration (the loop counter repeats after a rollback), history and control bit for
+    bit; with the tail fit, whose smoothing restarts after a rollback."""
+    g, p = setup()
+    # the halved step after a rollback falls below the default delta_t_min = 1e-5 here,
+    # which would end dmfe_run's loop (its while condition); dmfe_loop_iteration has none
+    p.delta_t_min = 1e-8
+    host = dm.Context(g, p, rows_hint=256)
+    dev = dm.Context(g, p, rows_hint=64)  # small arena: grows during the run
+    for c in (host, dev):
+        dm.set_rollback(c, True)
+        c.init_quench()
+        k = c.get_control()
+        c.set_control(1e-5, k["delta"], k["delta_old"], k["loop"], k["method"])  # the usual first step
+        if tail_fit:
+            dm.set_tail_fit(c, True)
+    ref = [host_records(host, 60)]
+    n, rec = dev.run(1e300, 60, corr_cap=60)
+    assert n == 60
+    got = [rec]
+    for spike in (1000.0, 3000.0):
+        for c in (host, dev):
+            k = c.get_control()
+            c.set_control(spike * k["delta_t"], k["delta"], k["delta_old"], k["loop"], k["method"])
+        T1 = host.T1
+        ref.append(host_records(host, 45))
+        assert host.T1 < T1 + 45  # rows were dropped
+        target = host.get_control()["loop"]
+        n, rec = dev.run(1e300, target, corr_cap=64)
+        assert n == 45, n
+        got.append(rec)
+    ref, got = np.vstack(ref), np.vstack(got)
+    assert np.array_equal(ref, got)
+    sa, sb = host.get_state(), dev.get_state()
+    for key in sa:
+        assert np.array_equal(sa[key], sb[key]), key
+    assert host.get_control() == dev.get_control()
+    if tail_fit:
+        assert dm.tail_fit_stats(host) == dm.tail_fit_stats(dev)
+    host.close()
+    dev.close()
