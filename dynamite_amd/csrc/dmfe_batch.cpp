@@ -75,6 +75,7 @@ struct Inst {
     Work W{};
     int split = 1;
     int gpu_variants = 0;  // the reference GPU path's interpolation (a context's device loop)
+    int use_serk2 = 0;     // the maintenance promotes to SERK2 levels (a context's device loop)
     int64_t cap = 0;
     TanArgs X{};  // tangents (dmfe_batch_tangent_init, or a context's device loop); X.K = 0: none
     History tH[kMaxTan] = {};  // the tangent histories this batch owns (dmfe_batch_tangent_init)
@@ -99,10 +100,13 @@ struct dmfe_batch {
     double* d_obs = nullptr;
     int obs_cap = 0;
     bool obs_on = false;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // 7- and 10-stage step sequences
-    int graph_steps[2] = {0, 0};
-    bool graph_obs[2] = {false, false};
-    bool graph_s0[2] = {false, false};  // the graph holds the stage-0 search launch
+    // captured step sequences by stage count (graph_slot): 7 and 10 stages, and the SERK2
+    // levels' 4, 8, ..., kMaxStepStages
+    static constexpr int kSlots = 2 + kMaxStepStages / 4;
+    hipGraphExec_t graph[kSlots] = {};
+    int graph_steps[kSlots] = {};
+    bool graph_obs[kSlots] = {};
+    bool graph_s0[kSlots] = {};  // the graph holds the stage-0 search launch
     int64_t steps_run = 0;
     bool borrowed = false;  // device_loop_run: tables, instance buffers and stream belong to a context
     // device loop: the correlation record of the last rollback iteration, recomputed after
@@ -147,13 +151,19 @@ int pull_ctl(dmfe_batch* b) {
 }
 
 int push_ctl(dmfe_batch* b) {
+    // a SERK2 level's embedded weights travel with the control word
+    for (Ctl& c : b->ctl)
+        if (c.method >= 3) {
+            const std::vector<double> w = serk2_coeffs(2 * (c.method - 2));
+            for (size_t k = 0; k < w.size() && k <= (size_t)kMaxStepStages; ++k) c.serk_b[k] = w[k];
+        }
     HIP_TRY(hipMemcpyAsync(b->d_ctl, b->ctl.data(), b->ctl.size() * sizeof(Ctl), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     return DMFE_OK;
 }
 
 void drop_graphs(dmfe_batch* b) {
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < dmfe_batch::kSlots; ++k) {
         if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
         b->graph[k] = nullptr;
         b->graph_steps[k] = 0;
@@ -233,12 +243,17 @@ int enqueue_step(dmfe_batch* b, int S) {
     return DMFE_OK;
 }
 
+int stages_of(int method) { return method >= 3 ? 4 * (method - 2) : (method == DMFE_RK54 ? 7 : 10); }
+
 int max_stages(const dmfe_batch* b) {
     int S = 0;
     for (const Ctl& c : b->ctl)
-        if (c.status == kRunning) S = std::max(S, c.method == DMFE_RK54 ? 7 : 10);
+        if (c.status == kRunning) S = std::max(S, stages_of(c.method));
     return S;
 }
+
+// graph of an S-stage step sequence: 7 -> 0, 10 -> 1, SERK2 S = 4k -> 1 + k
+int graph_slot(int S) { return S == 7 ? 0 : (S == 10 ? 1 : 1 + S / 4); }
 
 int build_graph(dmfe_batch* b, int which, int S, int steps) {
     const bool s0 = needs_stage0(b);
@@ -363,7 +378,11 @@ int maintenance(dmfe_batch* b, int i) {
     // last_rollback_loop stays -1000 unless rollbacks are on (simulation_runner.cpp:53,220)
     if (c.delta < dmax / 2 && c.loop - c.last_rollback > 1000) {
         c.dt *= 0.5;
-        c.method = DMFE_SSPRK104;
+        // SSPRK104, or with SERK2 on (a context's device loop) one level up from SSPRK104
+        // and SERK2 (init_SERK2(2 (init - 1)) sets init + 1, runge_kutta.cu:274), capped at
+        // the rmax table
+        if (!x.use_serk2 || c.method == DMFE_RK54) c.method = DMFE_SSPRK104;
+        else if (c.method < kMaxMethod) c.method += 1;
     }
     double rho_last = 0.0;
     HIP_TRY(hipMemcpyAsync(&rho_last, x.H.rho + (c.T1 - 1), sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -596,6 +615,7 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
     Inst& x = b->inst[0];
     x.gi = 0; x.prm = q.prm; x.M = q.M; x.H = q.H; x.W = q.W; x.cap = q.cap; x.X = q.X;
     x.gpu_variants = q.gpu_variants;
+    x.use_serk2 = q.use_serk2;
     const bool rec = corr && corr_cap > 0;
     if (rec != b->obs_on) b->obs_on = rec;  // build_graph re-captures when the recording flag changes
     Ctl& c = b->ctl[0];
@@ -629,7 +649,7 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
     while (!rc) {
         const int S = max_stages(b);
         if (S == 0) break;
-        const int which = S == 7 ? 0 : 1;
+        const int which = graph_slot(S);
         const int64_t before = b->ctl[0].loop;
         hipError_t e = hipSuccess;
         if (q.use_graph) {
@@ -1003,7 +1023,7 @@ extern "C" int dmfe_batch_run(dmfe_batch* b, int64_t max_steps, int use_graph, i
         if (S == 0) break;
         const int steps = (int)std::min<int64_t>(chunk, max_steps - done);
         if (use_graph && steps == chunk) {
-            const int which = S == 7 ? 0 : 1;
+            const int which = graph_slot(S);
             rc = build_graph(b, which, S, steps);
             if (rc) return rc;
             HIP_TRY(hipGraphLaunch(b->graph[which], b->stream));

@@ -1088,6 +1088,7 @@ static int run_device_loop(dmfe_ctx* c, double tmax, int64_t max_loop, double* c
         q.t_prop = t_prop;
         q.gpu_variants = c->gpu_variants ? 1 : 0;
         q.rollback = c->rollback ? 1 : 0;
+        q.use_serk2 = c->use_serk2 ? 1 : 0;
         q.last_rollback = c->last_rollback;
         double* cseg = (corr && done < corr_cap) ? corr + 3 * done : nullptr;
         int64_t seg_iters = 0;
@@ -1158,7 +1159,7 @@ long double cheb_U(int n, long double x) {
     return b;
 }
 
-std::vector<double> serk2_coeffs(int q) {
+std::vector<double> serk2_solve(int q) {
     const int m = 2, s = m * q, n = s + 1;
     const long double a2 = (long double)q / ((long double)s * s);
     const long double w0 = 1.0L + (19.0L / 20.0L) / ((long double)s * s);
@@ -1194,6 +1195,19 @@ std::vector<double> serk2_coeffs(int q) {
     return b;
 }
 
+}  // namespace
+
+// the weights of each level are solved once per process
+std::vector<double> dmfe::serk2_coeffs(int q) {
+    static std::mutex mu;
+    static std::map<int, std::vector<double>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(q);
+    if (it == cache.end()) it = cache.emplace(q, serk2_solve(q)).first;
+    return it->second;
+}
+
+namespace {
 // One SERK2 step (SERK2GPU, runge_kutta.cu:397-487, on this path's searches): stage n
 // searches the positions (stage 0 only on a fresh history), evaluates the RHS and the
 // stage launch of n+1 finishes it: g += dt alpha h (even n), g = 2 g - g_e + 2 dt alpha h
@@ -1363,9 +1377,9 @@ extern "C" int dmfe_set_loop_graphs(dmfe_ctx* c, int on) {
 
 extern "C" int dmfe_run(dmfe_ctx* c, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps) {
     if (!c || c->T1 < 1) return set_error(DMFE_ERR_ARG, "dmfe_run: no state");
-    // dense MFMA contexts, per-launch profiling, SERK2 and rollbacks with tangents keep the
-    // host-driven loop
-    if (!c->dense_mfma && !c->profiling && !c->use_serk2 && c->method < 3 && !(c->rollback && c->tanK))
+    // dense MFMA contexts, per-launch profiling, and tangents with SERK2 or rollbacks keep
+    // the host-driven loop (which reports that SERK2 steps carry no tangents)
+    if (!c->dense_mfma && !c->profiling && !(c->tanK && (c->rollback || c->use_serk2 || c->method >= 3)))
         return run_device_loop(c, tmax, max_loop, corr, corr_cap, steps);
     int64_t done = 0;
     double t = c->t1[c->T1 - 1];

@@ -1719,7 +1719,7 @@ __global__ __launch_bounds__(kRowThreads) void gather_rows_kernel(Tables T, Hist
 struct StepHead {
     int T1s, S, fc0, method;
     double dt, tprev, gt_err;
-    double gt[kMaxStages + 1];
+    double gt[kMaxStepStages + 1];
 };
 
 #pragma clang fp contract(off)
@@ -1749,11 +1749,14 @@ __device__ inline bool make_search_dev(const C& c, const InstDesc& d, int n, Sea
     } else {
         a.T1 = c.T1s + 1;
         a.mode = kSearchI;
-        if (ss && n >= 5 && n <= 7) {
+        // SSPRK104 reuses the positions of stages 2..4 at stages 5..7 (runge_kutta.cpp:60-110);
+        // SERK2 searches at every stage
+        const bool reuse = c.method == DMFE_SSPRK104;
+        if (reuse && n >= 5 && n <= 7) {
             a.mode = kSearchCopy;
             a.srcB1 = d.W.saveB1 + (size_t)(n - 5) * d.T.N;
             a.srcB2 = d.W.saveB2 + (size_t)(n - 5) * d.T.P;
-        } else if (ss && n >= 2 && n <= 4) {
+        } else if (reuse && n >= 2 && n <= 4) {
             a.saveB1 = d.W.saveB1 + (size_t)(n - 2) * d.T.N;
             a.saveB2 = d.W.saveB2 + (size_t)(n - 2) * d.T.P;
         }
@@ -1767,6 +1770,36 @@ __device__ inline bool make_search_dev(const C& c, const InstDesc& d, int n, Sea
         a.gpu_clamp = 1;
     }
     return true;
+}
+
+// Finish of SERK2 stage m (method >= 3: q = 2 (method - 2), S = 2q stages;
+// runge_kutta.cu:397-487 as the host-driven rk_step_serk2 sequences it): g += dt alpha h
+// at even m, g = 2 g - g_e + 2 dt alpha h at odd m, alpha = 1 / (4q), every stage
+// rewriting the row appended at stage 0; the embedded sum takes b_0 of the start row and
+// b_{m+1} of each stage.  gt[] holds the stage times the begin kernel formed.
+template <class C>
+__device__ inline void make_finish_serk_dev(const C& c, const InstDesc& d, int m, FinishArgs& f) {
+    const int S = c.S, q = S / 2;
+    const double dt = c.dt, alpha = 1.0 / (4 * q);
+    f = FinishArgs{};
+    f.mode = 1;
+    f.stage = m == 0 ? 0 : 1;
+    f.nterms = 1;
+    f.split = d.split;
+    f.lr = (m == 0) ? c.T1s - 1 : c.T1s;
+    f.g0row = (m == 0) ? c.T1s - 1 : c.T1s;  // the start row, then the previous stage's
+    f.wrow = c.T1s;
+    f.t_rhs = c.gt[m];
+    f.t_new = c.gt[m + 1];
+    f.serk = (m % 2 == 1) ? 2 : 1;
+    f.sa = (m % 2 == 1) ? 2 * dt * alpha : dt * alpha;
+    f.sb0 = c.serk_b[0];
+    f.sb = c.serk_b[m + 1];
+    f.tdiff = c.gt[m + 1] - c.gt[0];
+    f.rho_new = (c.T1s > 1) ? f.tdiff / (c.gt[0] - c.tprev) : 0.0;
+    f.last = (m == S - 1) ? 1 : 0;
+    if (f.last) f.gt_err = c.gt_err;
+    f.fbuf = (c.fc0 + searches_before(c, m + 1)) & 1;
 }
 
 
@@ -1832,19 +1865,42 @@ __device__ __forceinline__ void stage_role_body(const InstDesc& d, int role, int
     // trip) before they are tested.  The device loop (PRE, about two blocks per CU) also
     // keeps its finish arguments in registers across the block's global stores, which
     // could otherwise alias them; the many-instance batch keeps occupancy instead.
+    // SERK2 steps (method >= 3, up to kMaxStepStages stages) form the arguments here from
+    // the stage times; the other integrators' were written by the begin kernel
     if (role == -2) {
         if (n < 1) return;
         if constexpr (PRE) {
-            const int active = c.active, S = c.S;
-            const FinishArgs fa = c.fin[n - 1];
+            const int active = c.active, S = c.S, method = c.method;
+            if (method >= 3) {
+                FinishArgs fa;
+                if (active && n <= S) {
+                    make_finish_serk_dev(c, d, n - 1, fa);
+                    finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, fa, red);
+                }
+                return;
+            }
+            // (the index is clamped: a many-stage step of another instance launches more stages)
+            const FinishArgs fa = c.fin[min(n - 1, kMaxStages - 1)];
             if (active && n <= S) finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, fa, red);
         } else if (c.active && n <= c.S) {
-            finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
+            if (c.method >= 3) {
+                FinishArgs fa;
+                make_finish_serk_dev(c, d, n - 1, fa);
+                finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, fa, red);
+            } else {
+                finish_block<EPT, PRE>(d.T, d.H, d.W, d.M, c.fin[n - 1], red);
+            }
         }
         return;
     }
     const int active = c.active, S = c.S;
-    const SearchArgs a = c.srch[n];
+    SearchArgs a;
+    if (c.method >= 3) {
+        if (!active || n > S) return;
+        make_search_dev(c, d, n, a);
+    } else {
+        a = c.srch[min(n, kMaxStages)];
+    }
     if (!active || n > S || a.mode < 0) return;
     if (role == -1) search_theta_row<EPT>(d.T, d.H, d.W, a, lds, red);
     else search_phi2_row<EPT>(d.T, d.W, a, role, lds);

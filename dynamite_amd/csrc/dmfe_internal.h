@@ -30,6 +30,8 @@ constexpr int kMaxN = 2048;     // LDS sizing of the row kernels
 constexpr int kMaxStages = 10;  // SSPRK(10,4)
 constexpr int kSums = 8;        // quadratures per RHS row
 constexpr int kMaxMethod = 11;  // integrators 1 RK54, 2 SSPRK104, 3.. SERK2 levels (the rmax table)
+// stages of one step at the highest SERK2 level (q = 2 (kMaxMethod - 2), 2q stages)
+constexpr int kMaxStepStages = 4 * (kMaxMethod - 2);
 // RHS rows with N <= kTailMaxN keep the last two history rows (V[T1-2], D[T1-1]) in LDS
 // beside V[T1-1], so B2 targets in the last time interval read the block's LDS copy.
 constexpr int kTailMaxN = 1024;
@@ -285,13 +287,17 @@ struct Ctl {
     double rmax[11];
     // per-step scratch, set by the begin kernel
     int T1s, S, fc0, pad;
-    double gt[kMaxStages + 1];
+    double gt[kMaxStepStages + 1];  // gt[n]: time of the state the RHS of stage n reads; gt[S]: the step's end
     double tprev;       // t1grid[T1s - 2]
     double gt_err;      // |gt_S - gte|
     double rho_step;    // rho of the row the step writes (set with the last stage's arguments)
-    // launch arguments of this step, written by the begin kernel
+    // launch arguments of this step, written by the begin kernel (RK54 and SSPRK104; the
+    // stage launches of a SERK2 step form theirs from gt[] and serk_b)
     FinishArgs fin[kMaxStages];      // finish of stage m
     SearchArgs srch[kMaxStages + 1]; // searches of stage n (mode < 0: none)
+    // SERK2 (method >= 3): the embedded weights b_0..b_S of the level, set by the host
+    // whenever it hands the control word to the device
+    double serk_b[kMaxStepStages + 1];
 };
 
 struct InstDesc {
@@ -340,7 +346,10 @@ struct DeviceLoop {
     int gpu_variants;  // the reference GPU path's interpolation variants
     int rollback;      // rollbacks on error spikes (simulation_runner.cpp:280-303)
     int64_t last_rollback;  // in/out: last_rollback_loop
+    int use_serk2;     // the maintenance promotes SSPRK104 to SERK2 levels (dmfe_set_serk2)
 };
+// SERK2 embedded weights b_0..b_2q of level q (runge_kutta.cu:228-270), in long double
+std::vector<double> serk2_coeffs(int q);
 // *slot caches the loop's launch maps and captured graphs between calls (NULL at first;
 // released with device_loop_close).
 int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_loop, double* corr, int64_t corr_cap,

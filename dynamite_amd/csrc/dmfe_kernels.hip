@@ -69,7 +69,7 @@ __device__ __forceinline__ void begin_body(const InstDesc& d, int* go_lds, StepH
             const double g0 = t1[T1 - 1];
             const double tprev = T1 >= 2 ? t1[T1 - 2] : 0.0;
             const int t = method == DMFE_RK54 ? 0 : 1;
-            const int S = t == 0 ? 7 : 10;
+            const int S = method >= 3 ? 4 * (method - 2) : (t == 0 ? 7 : 10);
             StepHead& h = *sh;
             h.T1s = T1;
             h.S = S;
@@ -78,17 +78,36 @@ __device__ __forceinline__ void begin_body(const InstDesc& d, int* go_lds, StepH
             h.dt = dt;
             h.tprev = tprev;
             h.gt[0] = g0;
-            h.gt[1] = g0 + dt * cA[t][1][0] * 1.0;
-            for (int m = 1; m < S; ++m) {
-                const bool last = (m == S - 1);
-                double g = g0;
-                const int nt = last ? S : m + 1;
-                for (int j = 0; j < nt; ++j) g += dt * (last ? cB[t][j] : cA[t][m + 1][j]) * 1.0;
-                h.gt[m + 1] = g;
+            if (method >= 3) {
+                // SERK2 stage times (runge_kutta.cu:397-487): Euler steps of dt alpha at even
+                // stages, Chebyshev steps 2 g - g_e + 2 dt alpha at odd ones
+                const double alpha = 1.0 / (4 * (S / 2));
+                double g = g0, ge = g0;
+                for (int m = 0; m < S; ++m) {
+                    if (m == 0) {
+                        g = g0 + dt * alpha * 1.0;
+                    } else if (m % 2 == 0) {
+                        g = g + dt * alpha * 1.0;
+                    } else {
+                        g = 2 * g - ge + 2 * dt * alpha * 1.0;
+                        ge = g;
+                    }
+                    h.gt[m + 1] = g;
+                }
+                h.gt_err = fabs((g0 + dt) - h.gt[S]);
+            } else {
+                h.gt[1] = g0 + dt * cA[t][1][0] * 1.0;
+                for (int m = 1; m < S; ++m) {
+                    const bool last = (m == S - 1);
+                    double g = g0;
+                    const int nt = last ? S : m + 1;
+                    for (int j = 0; j < nt; ++j) g += dt * (last ? cB[t][j] : cA[t][m + 1][j]) * 1.0;
+                    h.gt[m + 1] = g;
+                }
+                double gte = g0;
+                for (int j = 0; j < S; ++j) gte += dt * cB2[t][j] * 1.0;
+                h.gt_err = fabs(h.gt[S] - gte);
             }
-            double gte = g0;
-            for (int j = 0; j < S; ++j) gte += dt * cB2[t][j] * 1.0;
-            h.gt_err = fabs(h.gt[S] - gte);
             c.T1s = T1;
             c.S = S;
             c.fc0 = fcur;
@@ -106,8 +125,10 @@ __device__ __forceinline__ void begin_body(const InstDesc& d, int* go_lds, StepH
     if (!go) return;
     const StepHead& h = *sh;
     const int k = threadIdx.x;
-    if (k <= h.S) make_search_dev(h, d, k, c.srch[k]);
-    else if (k >= 16 && k - 16 < h.S) make_finish_dev(h, d, k - 16, c.fin[k - 16]);
+    if (h.method < 3) {  // a SERK2 step's stage launches form their own arguments
+        if (k <= h.S) make_search_dev(h, d, k, c.srch[k]);
+        else if (k >= 16 && k - 16 < h.S) make_finish_dev(h, d, k - 16, c.fin[k - 16]);
+    }
     if (k == 0) c.active = 1;
 }
 

@@ -225,7 +225,8 @@ int dmfe_loop_iteration(dmfe_ctx* ctx, double* t_out);
  * of iterations run.  The loop runs with its step control on the device (stage
  * arguments, error norm and dt rule evaluated by kernels, 16 steps per host round
  * trip); results are bit-identical to repeated dmfe_loop_iteration calls.  Contexts
- * on dense grids, with per-launch profiling or SERK2 on use the host loop.  Rows of corr
+ * on dense grids, with per-launch profiling, or with tangents together with SERK2 or
+ * rollbacks use the host loop.  Rows of corr
  * are iterations: after a rollback the loop counter repeats values, as correlation.txt
  * does. */
 int dmfe_run(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* corr, int64_t corr_cap, int64_t* steps);
@@ -236,8 +237,8 @@ int dmfe_run(dmfe_ctx* ctx, double tmax, int64_t max_loop, double* corr, int64_t
  * time (method m >= 3 is SERK2 with q = 2 (m - 2): 2q stages of dt / (4q) in a Chebyshev
  * recursion, the error against the embedded combination sum_k b_k g_k; step cap
  * rmax[m - 1] / specRad) and a rollback steps SERK2 one level down.  Methods 3..11 can
- * also be set directly (dmfe_set_control).  Not with sensitivities; dmfe_run uses the
- * host-driven loop for SERK2. */
+ * also be set directly (dmfe_set_control).  Not with sensitivities; dmfe_run runs SERK2
+ * steps on its device-driven loop (the stage launches form each stage's arguments). */
 int dmfe_set_serk2(dmfe_ctx* ctx, int on);
 
 /* The rollback of the reference's GPU main loop (simulation_runner.cpp:280-303,

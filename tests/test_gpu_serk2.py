@@ -81,3 +81,54 @@ def test_main_loop_promotes_to_serk2_like_the_oracle():
         assert rel(sg[key], so[key]) <= 1e-9, key
     ctx.close()
     o.close()
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_device_loop_serk2_matches_host_loop_and_oracle(graphs):
+    """dmfe_run's device-driven loop through every SERK2 level: the maintenance every 60
+    loops promotes RK54 -> SSPRK104 -> SERK2(2) -> ... -> SERK2(18) (method 11, 36 stages,
+    the last level of the rmax table) over 660 loops.  The device loop is bit-identical to
+    the host-driven loop (history, control, per-iteration t), and both follow the oracle
+    with the identical t sequence."""
+    g = grid()
+    p = dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.8)
+    p.maint_every = 60
+    p.delta_max = 1e-3
+    p.delta_t_min = 1e-12  # each maintenance halves delta_t; dmfe_run stops below delta_t_min
+    L = 660
+    host, dev = dm.Context(g, p, rows_hint=64), dm.Context(g, p, rows_hint=64)
+    o = Oracle(g, delta_max=1e-3)
+    o.set_serk2(True)
+    o.set_maint_every(60)
+    for x in (host, dev):
+        dm.set_serk2(x, True)
+        x.init_quench()
+    o.init_quench()
+    dm.set_loop_graphs(dev, graphs)
+    c = host.get_control()
+    for x in (host, dev, o):
+        x.set_control(4e-5, c["delta"], c["delta_old"], c["loop"], c["method"])
+    th, to, methods = [], [], []
+    for k in range(L):
+        th.append(host.loop_iteration())
+        to.append(o.loop_iteration())
+        methods.append(host.get_control()["method"])
+    print(f"methods reached: {sorted(set(methods))}", flush=True)
+    assert methods[-1] == 11 and methods.count(11) >= 50
+    assert th == to
+    n, corr = dev.run(1e300, L, corr_cap=L)
+    assert n == L
+    assert np.array_equal(corr[:, 0], np.asarray(th))
+    ch, cd = host.get_control(), dev.get_control()
+    for key in ("delta_t", "delta", "delta_old", "loop", "method"):
+        assert ch[key] == cd[key], (key, ch[key], cd[key])
+    sh, sd, so = host.get_state(), dev.get_state(), o.state()
+    for key in sh:
+        assert np.array_equal(sh[key], sd[key]), key
+    assert np.array_equal(sd["t1grid"], so["t1grid"])
+    for key in ("QK", "QR", "dQK", "dQR", "rvec", "drvec"):
+        e = rel(sd[key], so[key])
+        print(f"device loop vs oracle after {L} loops: {key} rel err {e:.2e}", flush=True)
+        assert e <= 1e-10, key
+    for x in (host, dev, o):
+        x.close()
