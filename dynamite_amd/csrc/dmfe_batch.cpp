@@ -651,8 +651,11 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
         if (S == 0) break;
         const int which = graph_slot(S);
         const int64_t before = b->ctl[0].loop;
+        // iterations the loop counter still allows: a last partial chunk runs as direct
+        // launches, not as a replay whose surplus steps would each still launch every kernel
+        const int chunk = (int)std::min<int64_t>(kLoopChunk, std::max<int64_t>(max_loop - before, 1));
         hipError_t e = hipSuccess;
-        if (q.use_graph) {
+        if (q.use_graph && chunk == kLoopChunk) {
             rc = build_graph(b, which, S, kLoopChunk);
             if (rc) break;
             e = hipGraphLaunch(b->graph[which], b->stream);
@@ -661,7 +664,7 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
                 break;
             }
         } else {
-            for (int s = 0; s < kLoopChunk && rc == DMFE_OK; ++s) rc = enqueue_step(b, S);
+            for (int s = 0; s < chunk && rc == DMFE_OK; ++s) rc = enqueue_step(b, S);
             if (rc) break;
         }
         rc = pull_ctl(b);
