@@ -203,13 +203,17 @@ bool needs_stage0(const dmfe_batch* b) {
     return false;
 }
 
-// One batch step: begin | (stage n, rhs n) for n < S | stage S | end [| observables]
-int enqueue_step(dmfe_batch* b, int S) {
+// One batch step: begin | (stage n, rhs n) for n < S | stage S | end [| observables].
+// The device loop of a context (borrowed) issues step pos of `count` consecutive steps:
+// without observables, a step's end and the next step's begin are one launch, so only the
+// first step launches its begin and only the last its end.
+int enqueue_step(dmfe_batch* b, int S, int pos = 0, int count = 1) {
     const int n = n_inst(b);
     const bool s0 = needs_stage0(b);
     if (b->borrowed) {  // one context: descriptor by value, no launch maps
         const InstDesc& d = b->desc0;
-        launch_loop_begin(d, b->stream);
+        const bool joined = !b->obs_on;
+        if (pos == 0 || !joined) launch_loop_begin(d, b->stream);
         for (int k = 0; k < S; ++k) {
             if (k > 0 || s0) launch_loop_stage(d, k, b->stream);
             // with tangents the next stage launch runs their finish of stage k before the
@@ -220,8 +224,12 @@ int enqueue_step(dmfe_batch* b, int S) {
         // the tail fit runs after update() and before the loop counter moves
         // (simulation_runner.cpp:135-141)
         if (d.W.tf) launch_loop_tail_fit(d, b->inst[0].prm.delta_max, b->stream);
-        launch_loop_end(d, b->stream);
-        if (b->obs_on) launch_batch_obs(b->d_desc, n, b->d_obs, b->obs_cap, b->stream);
+        if (joined && pos + 1 < count) {
+            launch_loop_end_begin(d, b->stream);
+        } else {
+            launch_loop_end(d, b->stream);
+            if (b->obs_on) launch_batch_obs(b->d_desc, n, b->d_obs, b->obs_cap, b->stream);
+        }
         HIP_TRY(hipGetLastError());
         return DMFE_OK;
     }
@@ -267,7 +275,7 @@ int build_graph(dmfe_batch* b, int which, int S, int steps) {
     // while this stream captures; the captured region itself holds kernel launches only
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
     int rc = DMFE_OK;
-    for (int s = 0; s < steps && rc == DMFE_OK; ++s) rc = enqueue_step(b, S);
+    for (int s = 0; s < steps && rc == DMFE_OK; ++s) rc = enqueue_step(b, S, s, steps);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return set_error(DMFE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -664,7 +672,7 @@ int device_loop_run(dmfe_batch** slot, DeviceLoop& q, double tmax, int64_t max_l
                 break;
             }
         } else {
-            for (int s = 0; s < chunk && rc == DMFE_OK; ++s) rc = enqueue_step(b, S);
+            for (int s = 0; s < chunk && rc == DMFE_OK; ++s) rc = enqueue_step(b, S, s, chunk);
             if (rc) break;
         }
         rc = pull_ctl(b);

@@ -163,6 +163,16 @@ __global__ __launch_bounds__(kBeginThreads) void loop_begin_kernel(InstDesc d) {
     __shared__ StepHead sh;
     begin_body(d, &go, &sh);
 }
+
+// the end of one step and the begin of the next in one launch (consecutive steps of the
+// device loop that record no observables between them): lane 0 runs the end's control
+// and then the begin's, which reads what the end stored
+__global__ __launch_bounds__(kBeginThreads) void loop_end_begin_kernel(InstDesc d) {
+    __shared__ int go;
+    __shared__ StepHead sh;
+    if (threadIdx.x == 0) end_body(d);
+    begin_body(d, &go, &sh);
+}
 #pragma clang fp contract(on)
 
 }  // namespace
@@ -351,6 +361,10 @@ void launch_loop_begin(const InstDesc& d, hipStream_t s) {
 
 void launch_loop_end(const InstDesc& d, hipStream_t s) {
     hipLaunchKernelGGL(loop_end_kernel, dim3(1), dim3(64), 0, s, d);
+}
+
+void launch_loop_end_begin(const InstDesc& d, hipStream_t s) {
+    hipLaunchKernelGGL(loop_end_begin_kernel, dim3(1), dim3(kBeginThreads), 0, s, d);
 }
 
 void launch_loop_tail_fit(const InstDesc& d, double delta_max, hipStream_t s) {
