@@ -205,15 +205,14 @@ bool needs_stage0(const dmfe_batch* b) {
 
 // One batch step: begin | (stage n, rhs n) for n < S | stage S | end [| observables].
 // The device loop of a context (borrowed) issues step pos of `count` consecutive steps:
-// without observables, a step's end and the next step's begin are one launch, so only the
+// a step's end, its observables and the next step's begin are one launch, so only the
 // first step launches its begin and only the last its end.
 int enqueue_step(dmfe_batch* b, int S, int pos = 0, int count = 1) {
     const int n = n_inst(b);
     const bool s0 = needs_stage0(b);
     if (b->borrowed) {  // one context: descriptor by value, no launch maps
         const InstDesc& d = b->desc0;
-        const bool joined = !b->obs_on;
-        if (pos == 0 || !joined) launch_loop_begin(d, b->stream);
+        if (pos == 0) launch_loop_begin(d, b->stream);
         for (int k = 0; k < S; ++k) {
             if (k > 0 || s0) launch_loop_stage(d, k, b->stream);
             // with tangents the next stage launch runs their finish of stage k before the
@@ -224,8 +223,8 @@ int enqueue_step(dmfe_batch* b, int S, int pos = 0, int count = 1) {
         // the tail fit runs after update() and before the loop counter moves
         // (simulation_runner.cpp:135-141)
         if (d.W.tf) launch_loop_tail_fit(d, b->inst[0].prm.delta_max, b->stream);
-        if (joined && pos + 1 < count) {
-            launch_loop_end_begin(d, b->stream);
+        if (pos + 1 < count) {
+            launch_loop_end_begin(d, b->obs_on ? b->d_obs : nullptr, b->obs_cap, b->stream);
         } else {
             launch_loop_end(d, b->stream);
             if (b->obs_on) launch_batch_obs(b->d_desc, n, b->d_obs, b->obs_cap, b->stream);

@@ -1981,21 +1981,25 @@ __global__ __launch_bounds__(kRowThreads) void loop_rhs_tan_kernel(InstDesc d, i
 
 // Observables of the instances that just stepped: (t_prop, E(t_prop), QK(t,0)) per row
 // (simulation_runner.cpp:308-337), appended at row (loop - 1) % cap of obs[inst].
-__global__ __launch_bounds__(kStageThreads) void batch_obs_kernel(const InstDesc* __restrict__ D, double* obs, int cap) {
-    __shared__ double red[2 * (kStageThreads / kWave)];
-    const InstDesc& d = D[blockIdx.x];
+// the observables of instance `inst` of the batch (one block)
+__device__ __forceinline__ void obs_body(const InstDesc& d, int inst, double* obs, int cap, double* red) {
     const Ctl& c = *d.ctl;
     if (!c.active) return;  // did not step in this batch step
     const int row = c.T1 - 1;
     const double t = c.t_prop;
     const double E = row_scalar_body(d.T, d.H, d.M, row, t, 1, red);
     if (threadIdx.x == 0) {
-        double* o = obs + ((size_t)blockIdx.x * cap + (size_t)((c.loop - 1) % cap)) * 4;
+        double* o = obs + ((size_t)inst * cap + (size_t)((c.loop - 1) % cap)) * 4;
         o[0] = t;
         o[1] = E;
         o[2] = d.H.V[(size_t)row * d.T.N].x;
         o[3] = (double)c.loop;
     }
+}
+
+__global__ __launch_bounds__(kStageThreads) void batch_obs_kernel(const InstDesc* __restrict__ D, double* obs, int cap) {
+    __shared__ double red[2 * (kStageThreads / kWave)];
+    obs_body(D[blockIdx.x], blockIdx.x, obs, cap, red);
 }
 
 // ---- dense (B-spline) interpolation as an FP64 MFMA contraction --------------------

@@ -367,7 +367,8 @@ void launch_batch_rhs_tan(InstDesc* D, const int4* map, int nblocks, int n, int 
 // the step launches of one context's device loop, the instance passed by value
 void launch_loop_begin(const InstDesc& d, hipStream_t s);
 void launch_loop_end(const InstDesc& d, hipStream_t s);
-void launch_loop_end_begin(const InstDesc& d, hipStream_t s);  // end of a step + begin of the next
+// end of a step [+ its observables into obs] + begin of the next
+void launch_loop_end_begin(const InstDesc& d, double* obs, int obs_cap, hipStream_t s);
 void launch_loop_stage(const InstDesc& d, int n, hipStream_t s);  // with tangents: also their finish
 void launch_loop_rhs(const InstDesc& d, int n, int log_response, int G, size_t lds, hipStream_t s);
 // tail fit of one context's device loop after the step's last stage launch

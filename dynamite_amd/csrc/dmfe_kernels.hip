@@ -164,13 +164,19 @@ __global__ __launch_bounds__(kBeginThreads) void loop_begin_kernel(InstDesc d) {
     begin_body(d, &go, &sh);
 }
 
-// the end of one step and the begin of the next in one launch (consecutive steps of the
-// device loop that record no observables between them): lane 0 runs the end's control
-// and then the begin's, which reads what the end stored
-__global__ __launch_bounds__(kBeginThreads) void loop_end_begin_kernel(InstDesc d) {
+// the end of one step, its observables (obs non-null) and the begin of the next in one
+// launch (consecutive steps of the device loop): lane 0 runs the end's control, the block
+// the observables of the step, then the begin, which reads what the end stored
+__global__ __launch_bounds__(kStageThreads) void loop_end_begin_kernel(InstDesc d, double* obs, int cap) {
     __shared__ int go;
     __shared__ StepHead sh;
+    __shared__ double red[2 * (kStageThreads / kWave)];
     if (threadIdx.x == 0) end_body(d);
+    if (obs) {
+        __syncthreads();  // the end's control word is visible to the block
+        obs_body(d, 0, obs, cap, red);
+        __syncthreads();  // every lane has read it before the begin rewrites it
+    }
     begin_body(d, &go, &sh);
 }
 #pragma clang fp contract(on)
@@ -363,8 +369,8 @@ void launch_loop_end(const InstDesc& d, hipStream_t s) {
     hipLaunchKernelGGL(loop_end_kernel, dim3(1), dim3(64), 0, s, d);
 }
 
-void launch_loop_end_begin(const InstDesc& d, hipStream_t s) {
-    hipLaunchKernelGGL(loop_end_begin_kernel, dim3(1), dim3(kBeginThreads), 0, s, d);
+void launch_loop_end_begin(const InstDesc& d, double* obs, int obs_cap, hipStream_t s) {
+    hipLaunchKernelGGL(loop_end_begin_kernel, dim3(1), dim3(kStageThreads), 0, s, d, obs, obs_cap);
 }
 
 void launch_loop_tail_fit(const InstDesc& d, double delta_max, hipStream_t s) {
