@@ -6,7 +6,8 @@
     python -m dynamite_amd.rocprof_summary pmc <dir> [last_k]
         mean of every collected counter per kernel over its last `last_k` dispatches;
     python -m dynamite_amd.rocprof_summary steps <dir> [last_k]
-        the device loop's steps (dispatches from one loop_begin_kernel to the next) over the
+        the device loop's steps (dispatches from one step start - loop_begin_kernel, or
+        loop_end_begin_kernel, which ends a step and begins the next - to the next) over the
         last `last_k` complete steps: wall time per step, kernel-busy time per kernel, and
         the idle time between dispatches.
 <dir> is the rocprofv3 -d directory (searched recursively for *.db).
@@ -59,7 +60,7 @@ def steps(path, k):
     nm = "name" if "name" in cols else "kernel_name"
     rows = [(_short(n), a, b) for n, a, b in con.execute(f"select {nm}, start, end from kernels order by start")]
     rows = [r for r in rows if r[0].startswith("loop_")]
-    heads = [i for i, r in enumerate(rows) if r[0] == "loop_begin_kernel"]
+    heads = [i for i, r in enumerate(rows) if r[0] in ("loop_begin_kernel", "loop_end_begin_kernel")]
     # steps in which the instance ran (a launch after the loop stopped exits at once)
     spans = [(a, b) for a, b in zip(heads[:-1], heads[1:])
              if max((r[2] - r[1] for r in rows[a:b] if r[0].startswith(("loop_rhs", "loop_fused"))), default=0) > 4000][-k:]
