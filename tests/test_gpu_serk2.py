@@ -97,7 +97,7 @@ def test_device_loop_serk2_matches_host_loop_and_oracle(graphs):
     p.delta_t_min = 1e-12  # each maintenance halves delta_t; dmfe_run stops below delta_t_min
     L = 660
     host, dev = dm.Context(g, p, rows_hint=64), dm.Context(g, p, rows_hint=64)
-    o = Oracle(g, delta_max=1e-3)
+    o = Oracle(g, delta_max=1e-3, dt_min=1e-12)
     o.set_serk2(True)
     o.set_maint_every(60)
     for x in (host, dev):
@@ -132,3 +132,48 @@ def test_device_loop_serk2_matches_host_loop_and_oracle(graphs):
         assert e <= 1e-10, key
     for x in (host, dev, o):
         x.close()
+
+
+def test_device_loop_serk2_at_L512_matches_oracle():
+    """SERK2 on dmfe_run's device loop at the configuration's grid (L=512 poly-9, spline 5,
+    Tmax 1e5; p=3+4, lambda=0.5, Gamma=0.6): from a live quench state (300 GPU steps,
+    T1=301, taken into a context with delta_t_min 1e-12), 12 loops at SERK2(2) and 6 at
+    SERK2(6) against the oracle's main loop on the same state: identical t sequence and
+    control, history within 1e-10."""
+    g = dm.generate_grid(512, 1e5, 5, dm.INTERP_POLY, 9, 0)
+    prm = dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.6)
+    ctx = dm.Context(g, prm, rows_hint=400)
+    ctx.init_quench()
+    ctx.run(float("inf"), 300)
+    # the SERK2 steps' error norms exceed 2 delta_max on this early state, so delta_t
+    # shrinks by 0.9 a step; a lower delta_t_min keeps dmfe_run's loop condition open
+    p2 = dm.default_params(p=3, p2=4, lambda_=0.5, T0=1e50, Gamma=0.6)
+    p2.delta_t_min = 1e-12
+    st, c = ctx.get_state(), ctx.get_control()
+    ctx.close()
+    ctx = dm.Context(g, p2, rows_hint=400)
+    ctx.set_state(st["t1grid"], st["QK"], st["QR"], st["dQK"], st["dQR"], st["rvec"], st["drvec"], st["dtratio"])
+    ctx.set_control(c["delta_t"], c["delta"], c["delta_old"], c["loop"], c["method"])
+    o = Oracle(g, p=3, p2=4, lam=0.5, T0=1e50, Gamma=0.6, dt_min=1e-12)
+    o.set_history(st["t1grid"], st["dtratio"], st["rvec"], st["drvec"], st["QK"], st["QR"], st["dQK"], st["dQR"])
+    ctx.interpolate(dm.POS_BSEARCH)
+    o.interpolate(0)
+    for method, loops in ((3, 12), (5, 6)):
+        k = ctx.get_control()
+        for x in (ctx, o):
+            x.set_control(k["delta_t"], k["delta"], k["delta_old"], k["loop"], method)
+        to = [o.loop_iteration() for _ in range(loops)]
+        n, corr = ctx.run(float("inf"), k["loop"] + loops, corr_cap=loops)
+        assert n == loops
+        assert np.array_equal(corr[:, 0], np.asarray(to)), (method, corr[:, 0], to)
+        cg, co = ctx.get_control(), o.control()
+        for key in ("delta_t", "loop", "method"):
+            assert cg[key] == co[key], (method, key, cg[key], co[key])
+        sg, so = ctx.get_state(), o.state()
+        assert np.array_equal(sg["t1grid"], so["t1grid"])
+        for key in ("QK", "QR", "dQK", "dQR", "rvec", "drvec"):
+            e = rel(sg[key], so[key])
+            print(f"L=512 SERK2 method {method}: {key} rel err {e:.2e}", flush=True)
+            assert e <= 1e-10, (method, key)
+    ctx.close()
+    o.close()
